@@ -1,0 +1,176 @@
+/*
+ * lbcuda_amd.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann time step.
+ *
+ * The reference (copmat/LBcuda) has no plugin or FFI layer: its host program
+ * main.CUF launches CUDA-Fortran kernels with chevrons inside the step loop
+ * (main.CUF:125-308) and keeps all state in module globals
+ * (kernels_fluid.CUF:9-48, dimensions_m.CUF:318-347).  This header is the
+ * drop-in boundary that replaces those launches and uploads.  Every entry
+ * point below names the reference code it stands in for.  A Fortran host binds
+ * it with ISO_C_BINDING (host/lbcuda_amd_mod.f90); Python binds it with ctypes
+ * (lbcuda_amd/lib.py).
+ *
+ * Conventions at the boundary (reference: main.CUF:64,278, kernels_fluid.CUF):
+ *   - host arrays are Fortran-ordered (x fastest), interior nodes only
+ *     (1..nx, 1..ny, 1..nz), no halo;
+ *   - population arrays are (nx, ny, nz, 0:18);
+ *   - particle arrays follow pos(13, N) and forceAtoms(12, N) of
+ *     dimensions_m.CUF:355-362, stored as double;
+ *   - particle indices are 0-based in C, 1-based in Fortran;
+ *   - every function returns 0 on success or a positive LBM_ERR_* code.
+ */
+#ifndef LBCUDA_AMD_H
+#define LBCUDA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Population storage formats (the arithmetic is always FP32). */
+#define LBM_STORE_F32  0
+#define LBM_STORE_F16  1   /* f_i - w_i*rho0 stored as IEEE half  */
+#define LBM_STORE_BF16 2   /* f_i - w_i*rho0 stored as bfloat16   */
+
+/* Wetting of particle surface nodes. */
+#define LBM_WET_NONE    0  /* surface rho stays MINDENS (init_rho*_CG)              */
+#define LBM_WET_JANUS   1  /* compute_virtual_rho, kernels_fluid_PART.CUF:299-352   */
+#define LBM_WET_NEUTRAL 2  /* compute_densities_wall, kernels_fluid_PART.CUF:267-297 */
+
+/* Initial-state types: input.dat line 4 (Tests/01/input.dat:4, main.CUF:2531-2608). */
+#define LBM_INIT_DROPLET   1   /* setup2          kernels_fluid_CG.CUF:140-201 */
+#define LBM_INIT_ZPLANES   2   /* setup2_zplanes  kernels_fluid_CG.CUF:203-256 */
+#define LBM_INIT_RHO       3   /* rho from host + setupPops  main.CUF:2561-2568 */
+#define LBM_INIT_POPS      4   /* rho and pops from host     main.CUF:2570-2577 */
+#define LBM_INIT_CYLINDER  5   /* setupCyl        kernels_fluid_CG.CUF:258-311 */
+#define LBM_INIT_RANDOM    6   /* host random rho + setupPops main.CUF:2582-2595 */
+#define LBM_INIT_VELFIELD  7   /* rho and per-node velocity from host (no reference
+                                  counterpart: needed for Taylor-Green starts)     */
+
+/* Error codes (the reference stores __LINE__ in stop_d and aborts). */
+#define LBM_OK               0
+#define LBM_ERR_ARG          1   /* bad argument / config                      */
+#define LBM_ERR_ALLOC        2   /* device or host allocation failed           */
+#define LBM_ERR_HIP          3   /* HIP runtime error                          */
+#define LBM_ERR_RHO_RANGE    4   /* kernels_fluid_CG.CUF:438-441 (init_rhoR_CG) */
+#define LBM_ERR_BIG_FORCE    5   /* kernels_fluid_PART.CUF:1532-1537 (nve_lf)   */
+#define LBM_ERR_EXTREME_VEL  6   /* kernels_fluid_PART.CUF:1544-1548 (nve_lf)   */
+#define LBM_ERR_OUT_OF_BOX   7   /* kernels_fluid_PART.CUF:1581-1598 (nve_lf)   */
+#define LBM_ERR_PART_SETUP   8   /* main.CUF:1207-1221 (SetupParts checks)      */
+#define LBM_ERR_STATE        9   /* call order (e.g. step before init)          */
+#define LBM_ERR_BATCH       10   /* batched launch configuration                */
+
+/*
+ * Run-time configuration.  The reference fixes most of these at compile time
+ * (dimensions_m.CUF:7, 50, 59-84, 103-108; defines.h:10-24); here they are
+ * run-time so one library serves every box, tile and precision.
+ */
+typedef struct lbm_config {
+  int32_t nx, ny, nz;        /* box (glx, gly, glz), periodic on all axes        */
+  float   tau_r, tau_b;      /* relaxation times tauR, tauB (dimensions_m:103-104)*/
+  float   sigma;             /* surface tension sigma_CG (dimensions_m:50)        */
+  float   beta;              /* recolouring beta_CG (dimensions_m:50)             */
+  int32_t storage;           /* LBM_STORE_*                                       */
+  float   store_rho0;        /* rho0 of the shifted low-precision storage         */
+  float   min_rho, max_rho;  /* density range guard (kernels_fluid.CUF:35)        */
+  /* particles (dimensions_m.CUF:55-84) */
+  int32_t with_particles;
+  int32_t n_particles;
+  int32_t lrotate;
+  float   radius;            /* rdim                                              */
+  float   rmass;             /* 1/mass; 0 = prescribed constant velocity           */
+  float   max_vel;           /* MAX_PARTICLE_VEL                                  */
+  float   ext_force[3];      /* ext_fxx..ext_fzz                                   */
+  float   ext_torque[3];     /* ext_tqx..ext_tqz                                   */
+  int32_t wetting;           /* LBM_WET_*                                          */
+  int32_t hertz;             /* withHz                                             */
+  float   hz_k, hz_rmin, hz_rcap;
+  int32_t lubrication;       /* llubrication (applied to the FP64 sums)            */
+  float   lub_k, lub_rmin, lub_rcap;   /* cutoff gap, minimum gap (lattice units) */
+  int32_t closure;           /* learned near-contact closure (no reference part)   */
+  int32_t reserved[8];
+} lbm_config;
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Fill *cfg with the reference defaults (dimensions_m.CUF:50-84, 103-108,
+ * kernels_fluid.CUF:35) for an n x n x n box. */
+void lbm_config_default(lbm_config* cfg, int32_t n);
+
+/* Allocate device state for one box.  Replaces setupGPUgridAndAlloc
+ * (main.CUF:2133-2407) and SetConstants (main.CUF:1043-1162). */
+int lbm_create(const lbm_config* cfg, lbm_ctx** out);
+
+/* Release everything lbm_create allocated. */
+int lbm_destroy(lbm_ctx* ctx);
+
+/* Initial fluid state.  Replaces setupRho_Pops (main.CUF:2531-2608):
+ * init_type is an LBM_INIT_* value; u0 is (vx,vy,vz) of input.dat line 5;
+ * rho_r/rho_b are (nx,ny,nz) host arrays for LBM_INIT_RHO/RANDOM/VELFIELD/POPS
+ * (else NULL); vel is (3,nx,ny,nz) for LBM_INIT_VELFIELD (else NULL). */
+int lbm_init_fields(lbm_ctx* ctx, int32_t init_type, const float u0[3],
+                    const float* rho_r, const float* rho_b, const float* vel);
+
+/* Populations for LBM_INIT_POPS (read_and_setPops, main.CUF:2481-2529):
+ * pops_r/pops_b are (nx,ny,nz,19) host arrays. */
+int lbm_set_pops(lbm_ctx* ctx, const float* pops_r, const float* pops_b);
+
+/* Particle state.  Replaces read_input_atom + SetupParts + init_isfluid_CG at
+ * step 0 (main.CUF:1187-1300, 2596-2605).  pos is (13,n): x(3), v(3), omega(3),
+ * q(4); the same values seed both time levels as the reference does. */
+int lbm_set_particles(lbm_ctx* ctx, int32_t n, const double* pos);
+
+/* Advance nsteps time steps.  Replaces one pass of the do-step loop body
+ * (main.CUF:127-278) per step, in the same order.  Returns the first error
+ * raised by any step (LBM_ERR_RHO_RANGE ...), polled once at the end. */
+int lbm_step(lbm_ctx* ctx, int32_t nsteps);
+
+/* Post-streaming populations of colour 0 (red) or 1 (blue), (nx,ny,nz,19),
+ * as float (the state buffer a reference OutputVTK would copy, main.CUF:1823). */
+int lbm_get_pops(lbm_ctx* ctx, int32_t color, float* out);
+
+/* Density arrays rhoR_d / rhoB_d as they stand (lagging: last computed by the
+ * density pass and wetting of the last step), (nx,ny,nz). */
+int lbm_get_rho(lbm_ctx* ctx, float* rho_r, float* rho_b);
+
+/* Host moments of the current state (moments2Fl, write_output.f90:975-1041):
+ * rho_r, rho_b, phase (nx,ny,nz) and vel (3,nx,ny,nz); any may be NULL. */
+int lbm_get_moments(lbm_ctx* ctx, float* rho_r, float* rho_b, float* vel,
+                    float* phase);
+
+/* Node-type flags of the current step (myfluid_d(:,:,:,flip)), (nx,ny,nz):
+ * 1 fluid, 2 surface (spherelist), 4 inside (spheredead), 5 centre. */
+int lbm_get_flags(lbm_ctx* ctx, int8_t* out);
+
+/* Particle state at the current time level: pos (13,n) as lbm_set_particles,
+ * force (12,n): F(3), F_bb(3), T(3), T_bb(3) as forceAtoms_d after the last
+ * time_step_force_particles (kernels_fluid_PART.CUF:1266-1382). */
+int lbm_get_particles(lbm_ctx* ctx, double* pos, double* force);
+
+/* Global sums in FP64 (no reference counterpart; mass and momentum audit):
+ * out[0]=sum rhoR, out[1]=sum rhoB, out[2..4]=fluid momentum. */
+int lbm_get_sums(lbm_ctx* ctx, double out[5]);
+
+/* Error state: code (LBM_ERR_*), step at which it was raised, message. */
+int lbm_status(lbm_ctx* ctx, int32_t* code, int32_t* step, char* msg,
+               int32_t msg_len);
+
+/* Kernel timing with HIP events on the library's stream.  on != 0 brackets
+ * every launch of the fluid collide-stream kernel with events; get returns
+ * the summed milliseconds and the number of launches since enabling. */
+int lbm_kernel_timing(lbm_ctx* ctx, int32_t on);
+int lbm_get_kernel_timing(lbm_ctx* ctx, double* total_ms, int64_t* launches);
+
+/* Name of the collide-stream kernel variant in use and its algorithmic bytes
+ * per lattice update (2 colours x 19 pops x read+write x bytes/value + flag). */
+int lbm_kernel_info(lbm_ctx* ctx, char* name, int32_t name_len,
+                    double* bytes_per_lu);
+
+/* Library build identification (gfx target, storage formats). */
+const char* lbm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBCUDA_AMD_H */

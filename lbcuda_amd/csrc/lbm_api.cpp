@@ -1,0 +1,528 @@
+/*
+ * lbm_api.cpp -- host side of the C ABI declared in include/lbcuda_amd.h.
+ *
+ * Owns the device memory of one box and sequences the kernels of one time
+ * step in the order of the reference step loop (main.CUF:127-278):
+ *   [particles] flags from positions, erase (init_isfluid_CG, del_fluid_CG)
+ *   densities + phase field              (init_rhoB_CG, init_rhoR_CG)
+ *   [particles] fill, wetting            (make_fluid_CG, compute_*_rho)
+ *   collide + stream (+ bounce-back copy) (time_step_CG ... stream_CG_z)
+ *   [particles] link forces + moving wall, pair forces, rigid-body update
+ *                                        (partBB_CG*, inter_force_particles,
+ *                                         time_step_force_particles)
+ * All launches go to one stream; the host synchronises only when it reads
+ * something back or polls the error word at the end of lbm_step.
+ */
+#include "lbm_common.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+int lbm_particles_upload_belt2();
+
+namespace {
+
+int hip_ok(hipError_t e) { return e == hipSuccess ? 0 : LBM_ERR_HIP; }
+
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+void set_host_err(lbm_ctx* c, int code, const char* msg) {
+  if (c->host_err == 0) {
+    c->host_err = code;
+    c->host_err_step = c->step;
+    std::snprintf(c->host_msg, sizeof c->host_msg, "%s", msg);
+  }
+}
+
+template <class T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return 0;
+  return hipMalloc((void**)p, sizeof(T) * count) == hipSuccess ? 0 : LBM_ERR_ALLOC;
+}
+
+/* spherical_template (main.CUF:1990-2130): non-fluid offsets of one particle */
+void build_template(float rdim, int* rmax_out, std::vector<int8_t>& tmpl) {
+  const float sqrcut = rdim * rdim;
+  const int rmin = (int)std::floor(rdim);
+  const int rmax = (int)(rdim + 1.0f);
+  const int w = 2 * rmax + 1;
+  std::vector<int8_t> s((size_t)w * w * w, lbm::FL_FLUID);
+  auto S = [&](int i, int j, int k) -> int8_t& { return s[(size_t)((i + rmax) + w * ((j + rmax) + w * (k + rmax)))]; };
+  for (int k = -rmax; k <= rmax; ++k)
+    for (int j = -rmax; j <= rmax; ++j)
+      for (int i = -rmax; i <= rmax; ++i)
+        if ((float)(i * i + j * j + k * k) <= sqrcut) S(i, j, k) = lbm::FL_DEAD;
+  for (int k = -rmin; k <= rmin; ++k)
+    for (int j = -rmin; j <= rmin; ++j)
+      for (int i = -rmin; i <= rmin; ++i)
+        if (S(i, j, k) == lbm::FL_DEAD)
+          for (int l = 1; l < 19; ++l)
+            if (S(i + lbm::EX(l), j + lbm::EY(l), k + lbm::EZ(l)) == lbm::FL_FLUID) {
+              S(i, j, k) = lbm::FL_LIST;
+              break;
+            }
+  S(0, 0, 0) = lbm::FL_CM;
+  tmpl.clear();
+  for (int k = -rmax; k <= rmax; ++k)
+    for (int j = -rmax; j <= rmax; ++j)
+      for (int i = -rmax; i <= rmax; ++i)
+        if (S(i, j, k) != lbm::FL_FLUID) {
+          tmpl.push_back((int8_t)i); tmpl.push_back((int8_t)j); tmpl.push_back((int8_t)k);
+          tmpl.push_back(S(i, j, k));
+        }
+  *rmax_out = rmax;
+}
+
+/* offsets with i^2+j^2+k^2 <= rmax^2 (the distance tests of del_fluid,
+ * make_fluid: <= sphereMax; partBB_CG, partBB_CG_p2n2: < sphereMax) */
+void build_ball(int rmax, std::vector<int8_t>& ball) {
+  ball.clear();
+  const int smax = rmax * rmax;
+  for (int k = -rmax; k <= rmax; ++k)
+    for (int j = -rmax; j <= rmax; ++j)
+      for (int i = -rmax; i <= rmax; ++i) {
+        int d = i * i + j * j + k * k;
+        if (d <= smax) {
+          ball.push_back((int8_t)i); ball.push_back((int8_t)j); ball.push_back((int8_t)k);
+          ball.push_back((int8_t)(d < smax ? 1 : 0));
+        }
+      }
+}
+
+int restamp_initial(lbm_ctx* c) {
+  /* step-0 node flags (setupRho_Pops, main.CUF:2596-2605): the current level
+     from the positions, the other level stays all fluid. */
+  int rc = 0;
+  for (int b = 0; b < 2; ++b) {
+    rc |= hip_ok(hipMemsetAsync(c->d_key[b], 0xff, sizeof(uint32_t) * c->N, c->stream));
+    std::vector<int> inv((size_t)3 * c->np, INT32_MIN);
+    rc |= hip_ok(hipMemcpyAsync(c->d_stamp[b], inv.data(), sizeof(int) * inv.size(), hipMemcpyHostToDevice, c->stream));
+    rc |= hip_ok(hipStreamSynchronize(c->stream));
+  }
+  if (rc) return LBM_ERR_HIP;
+  return lbm_particles_flags(c, c->cur);
+}
+
+}  // namespace
+
+extern "C" {
+
+void lbm_config_default(lbm_config* cfg, int32_t n) {
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->nx = cfg->ny = cfg->nz = n;
+  cfg->tau_r = cfg->tau_b = 1.0f;
+  cfg->sigma = 0.1f;
+  cfg->beta = 0.999f;
+  cfg->storage = LBM_STORE_F32;
+  cfg->store_rho0 = 0.5f;
+  cfg->min_rho = -10.0f;
+  cfg->max_rho = 10.0f;
+  cfg->radius = 5.5f;
+  cfg->rmass = 1.0f / 1500.0f;
+  cfg->wetting = LBM_WET_JANUS;
+  cfg->hertz = 1;
+  cfg->hz_k = 10.0f; cfg->hz_rmin = 10.5f; cfg->hz_rcap = 9.0f;
+  cfg->lub_k = 10.0f; cfg->lub_rmin = 10.0f; cfg->lub_rcap = 9.0f;
+}
+
+const char* lbm_version(void) {
+  return "lbcuda_amd 0.1 gfx950 (storage: f32, f16-shifted, bf16-shifted)";
+}
+
+int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
+  if (!cfg || !out) return LBM_ERR_ARG;
+  *out = nullptr;
+  if (cfg->nx < 3 || cfg->ny < 3 || cfg->nz < 3) return LBM_ERR_ARG;
+  if (cfg->tau_r <= 0.5f || cfg->tau_b <= 0.5f) return LBM_ERR_ARG;
+  if (cfg->storage < LBM_STORE_F32 || cfg->storage > LBM_STORE_BF16) return LBM_ERR_ARG;
+  if (cfg->with_particles && (cfg->radius < 1.0f || cfg->radius > 30.0f)) return LBM_ERR_ARG;
+  lbm_ctx* c = new (std::nothrow) lbm_ctx;
+  if (!c) return LBM_ERR_ALLOC;
+  std::memset((void*)c, 0, sizeof *c);
+  c->cfg = *cfg;
+  c->nx = cfg->nx; c->ny = cfg->ny; c->nz = cfg->nz;
+  c->N = (size_t)c->nx * c->ny * c->nz;
+  c->storage = cfg->storage;
+  c->esize = cfg->storage == LBM_STORE_F32 ? 4 : 2;
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return LBM_ERR_HIP; }
+  for (int b = 0; b < 2; ++b) {
+    rc |= hipMalloc(&c->pop[b], c->esize * 38 * c->N) == hipSuccess ? 0 : LBM_ERR_ALLOC;
+    rc |= dalloc(&c->flags[b], c->N);
+  }
+  rc |= dalloc(&c->rho_r, c->N);
+  rc |= dalloc(&c->rho_b, c->N);
+  rc |= dalloc(&c->phi, c->N);
+  rc |= dalloc(&c->d_status, 1);
+  if (!rc) rc |= hip_ok(hipMemset(c->d_status, 0, sizeof(lbm::DevStatus)));
+  if (!rc && cfg->with_particles) {
+    std::vector<int8_t> tmpl, ball;
+    build_template(cfg->radius, &c->rmax, tmpl);
+    build_ball(c->rmax, ball);
+    c->ntmpl = (int)(tmpl.size() / 4);
+    c->nball = (int)(ball.size() / 4);
+    rc |= dalloc(&c->d_tmpl, tmpl.size());
+    rc |= dalloc(&c->d_ball, ball.size());
+    for (int b = 0; b < 2; ++b) rc |= dalloc(&c->d_key[b], c->N);
+    rc |= dalloc(&c->d_claim, c->N);
+    if (!rc) {
+      rc |= hip_ok(hipMemcpy(c->d_tmpl, tmpl.data(), tmpl.size(), hipMemcpyHostToDevice));
+      rc |= hip_ok(hipMemcpy(c->d_ball, ball.data(), ball.size(), hipMemcpyHostToDevice));
+      for (int b = 0; b < 2; ++b) rc |= hip_ok(hipMemset(c->d_key[b], 0xff, sizeof(uint32_t) * c->N));
+      rc |= hip_ok(hipMemset(c->d_claim, 0xff, sizeof(int) * c->N));
+      rc |= lbm_particles_upload_belt2();
+    }
+  }
+  if (rc) {
+    lbm_destroy(c);
+    return rc & LBM_ERR_ALLOC ? LBM_ERR_ALLOC : LBM_ERR_HIP;
+  }
+  *out = c;
+  return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx* c) {
+  if (!c) return LBM_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int b = 0; b < 2; ++b) {
+    dfree(c->pop[b]);
+    dfree(c->flags[b]);
+    dfree(c->d_pos[b]);
+    dfree(c->d_force[b]);
+    dfree(c->d_stamp[b]);
+    dfree(c->d_key[b]);
+  }
+  dfree(c->rho_r); dfree(c->rho_b); dfree(c->phi);
+  dfree(c->d_status);
+  dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
+  dfree(c->d_claim); dfree(c->d_tmpl); dfree(c->d_ball);
+  for (int e = 0; e < 2 * c->ev_cap; ++e) (void)hipEventDestroy(c->ev_pairs[e]);
+  delete[] c->ev_pairs;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return LBM_OK;
+}
+
+int lbm_init_fields(lbm_ctx* c, int32_t type, const float u0[3], const float* rho_r, const float* rho_b,
+                    const float* vel) {
+  if (!c) return LBM_ERR_ARG;
+  if (type < LBM_INIT_DROPLET || type > LBM_INIT_VELFIELD) return LBM_ERR_ARG;
+  const bool host_rho = type == LBM_INIT_RHO || type == LBM_INIT_RANDOM || type == LBM_INIT_VELFIELD ||
+                        type == LBM_INIT_POPS;
+  if (host_rho && (!rho_r || !rho_b)) return LBM_ERR_ARG;
+  if (type == LBM_INIT_VELFIELD && !vel) return LBM_ERR_ARG;
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  if (!u0) u0 = zero;
+  int rc = 0;
+  c->cur = 0;
+  c->step = 0;
+  rc |= hip_ok(hipMemsetAsync(c->d_status, 0, sizeof(lbm::DevStatus), c->stream));
+  rc |= hip_ok(hipMemsetAsync(c->rho_r, 0, sizeof(float) * c->N, c->stream));
+  rc |= hip_ok(hipMemsetAsync(c->rho_b, 0, sizeof(float) * c->N, c->stream));
+  if (host_rho) {
+    rc |= hip_ok(hipMemcpyAsync(c->rho_r, rho_r, sizeof(float) * c->N, hipMemcpyHostToDevice, c->stream));
+    rc |= hip_ok(hipMemcpyAsync(c->rho_b, rho_b, sizeof(float) * c->N, hipMemcpyHostToDevice, c->stream));
+  }
+  float* d_vel = nullptr;
+  if (type == LBM_INIT_VELFIELD) {
+    rc |= dalloc(&d_vel, 3 * c->N);
+    if (!rc) rc |= hip_ok(hipMemcpyAsync(d_vel, vel, sizeof(float) * 3 * c->N, hipMemcpyHostToDevice, c->stream));
+  }
+  if (!rc) rc |= lbm_launch_init_eq(c, type, u0, d_vel);
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  dfree(d_vel);
+  if (rc) return rc;
+  c->host_err = 0;
+  c->initialised = true;
+  if (c->np > 0) rc = restamp_initial(c);
+  if (!rc) rc = hip_ok(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int lbm_set_pops(lbm_ctx* c, const float* pr, const float* pb) {
+  if (!c || !pr || !pb) return LBM_ERR_ARG;
+  float *dr = nullptr, *db = nullptr;
+  int rc = dalloc(&dr, 19 * c->N) | dalloc(&db, 19 * c->N);
+  if (!rc) {
+    rc |= hip_ok(hipMemcpyAsync(dr, pr, sizeof(float) * 19 * c->N, hipMemcpyHostToDevice, c->stream));
+    rc |= hip_ok(hipMemcpyAsync(db, pb, sizeof(float) * 19 * c->N, hipMemcpyHostToDevice, c->stream));
+    if (!rc) rc |= lbm_launch_pack_pops(c, dr, db);
+    rc |= hip_ok(hipStreamSynchronize(c->stream));
+  }
+  dfree(dr);
+  dfree(db);
+  return rc;
+}
+
+int lbm_set_particles(lbm_ctx* c, int32_t n, const double* pos) {
+  if (!c || !pos || n <= 0) return LBM_ERR_ARG;
+  if (!c->cfg.with_particles) return LBM_ERR_ARG;
+  if (n > (1 << 28)) return LBM_ERR_ARG;
+  /* SetupParts sanity checks, main.CUF:1204-1222 */
+  const double R = c->cfg.radius;
+  if (c->nx < 2 * R || c->ny < 2 * R || c->nz < 2 * R) return LBM_ERR_PART_SETUP;
+  for (int a = 0; a < n; ++a) {
+    const double* p = pos + 13 * (size_t)a;
+    if (p[0] < 1 || p[0] > c->nx || p[1] < 1 || p[1] > c->ny || p[2] < 1 || p[2] > c->nz) return LBM_ERR_PART_SETUP;
+    for (int b = a + 1; b < n; ++b) {
+      const double* q = pos + 13 * (size_t)b;
+      double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+      if (std::sqrt(dx * dx + dy * dy + dz * dz) < 2 * R + 1) return LBM_ERR_PART_SETUP;
+    }
+  }
+  if (n != c->np) {
+    for (int b = 0; b < 2; ++b) {
+      dfree(c->d_pos[b]); dfree(c->d_force[b]); dfree(c->d_stamp[b]);
+      c->d_pos[b] = nullptr; c->d_force[b] = nullptr; c->d_stamp[b] = nullptr;
+    }
+    dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
+    c->d_myf = c->d_myf2 = c->d_myt = c->d_myt2 = nullptr;
+    int rc = 0;
+    for (int b = 0; b < 2; ++b) {
+      rc |= dalloc(&c->d_pos[b], (size_t)13 * n);
+      rc |= dalloc(&c->d_force[b], (size_t)12 * n);
+      rc |= dalloc(&c->d_stamp[b], (size_t)3 * n);
+    }
+    rc |= dalloc(&c->d_myf, (size_t)3 * n);
+    rc |= dalloc(&c->d_myf2, (size_t)3 * n);
+    rc |= dalloc(&c->d_myt, (size_t)3 * n);
+    rc |= dalloc(&c->d_myt2, (size_t)3 * n);
+    if (rc) { c->np = 0; return LBM_ERR_ALLOC; }
+    c->np = n;
+  }
+  /* read_input_atom (main.CUF:1300-1340): both levels start equal; forces
+     start at the external force/torque, bounce-back parts at zero */
+  const lbm_config& g = c->cfg;
+  std::vector<double> f12((size_t)12 * n, 0.0), m3((size_t)3 * n, 0.0), e3((size_t)3 * n), t3((size_t)3 * n);
+  for (int a = 0; a < n; ++a)
+    for (int d = 0; d < 3; ++d) {
+      f12[12 * (size_t)a + d] = g.ext_force[d];
+      f12[12 * (size_t)a + 6 + d] = g.lrotate ? g.ext_torque[d] : 0.0;
+      e3[3 * (size_t)a + d] = g.ext_force[d];
+      t3[3 * (size_t)a + d] = g.lrotate ? g.ext_torque[d] : 0.0;
+    }
+  int rc = 0;
+  for (int b = 0; b < 2; ++b) {
+    rc |= hip_ok(hipMemcpy(c->d_pos[b], pos, sizeof(double) * 13 * n, hipMemcpyHostToDevice));
+    rc |= hip_ok(hipMemcpy(c->d_force[b], f12.data(), sizeof(double) * f12.size(), hipMemcpyHostToDevice));
+  }
+  rc |= hip_ok(hipMemcpy(c->d_myf, m3.data(), sizeof(double) * m3.size(), hipMemcpyHostToDevice));
+  rc |= hip_ok(hipMemcpy(c->d_myt, m3.data(), sizeof(double) * m3.size(), hipMemcpyHostToDevice));
+  rc |= hip_ok(hipMemcpy(c->d_myf2, e3.data(), sizeof(double) * e3.size(), hipMemcpyHostToDevice));
+  rc |= hip_ok(hipMemcpy(c->d_myt2, t3.data(), sizeof(double) * t3.size(), hipMemcpyHostToDevice));
+  if (rc) return rc;
+  if (c->initialised) {
+    rc = restamp_initial(c);
+    if (!rc) rc = hip_ok(hipStreamSynchronize(c->stream));
+  }
+  return rc;
+}
+
+int lbm_step(lbm_ctx* c, int32_t nsteps) {
+  if (!c || nsteps < 0) return LBM_ERR_ARG;
+  if (!c->initialised) return LBM_ERR_STATE;
+  const bool parts = c->np > 0;
+  int rc = 0;
+  for (int s = 0; s < nsteps && !rc; ++s) {
+    c->step += 1;
+    if (parts) {
+      rc |= lbm_particles_flags(c, c->cur);
+      rc |= lbm_particles_erase(c);
+    }
+    rc |= lbm_launch_density(c);
+    if (parts) {
+      rc |= lbm_particles_fill(c);
+      rc |= lbm_particles_wetting(c);
+    }
+    rc |= lbm_launch_collide(c);
+    if (parts) {
+      rc |= lbm_particles_bounce(c);
+      rc |= lbm_particles_integrate(c);
+    }
+    c->cur = 1 - c->cur;
+  }
+  if (rc) {
+    set_host_err(c, LBM_ERR_HIP, "kernel launch failed");
+    return LBM_ERR_HIP;
+  }
+  lbm::DevStatus st;
+  if (hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    set_host_err(c, LBM_ERR_HIP, "device error while stepping");
+    return LBM_ERR_HIP;
+  }
+  return st.code;
+}
+
+int lbm_get_pops(lbm_ctx* c, int32_t color, float* out) {
+  if (!c || !out || color < 0 || color > 1) return LBM_ERR_ARG;
+  float* d = nullptr;
+  int rc = dalloc(&d, 19 * c->N);
+  if (!rc) {
+    rc |= lbm_launch_unpack_pops(c, color, d);
+    rc |= hip_ok(hipMemcpyAsync(out, d, sizeof(float) * 19 * c->N, hipMemcpyDeviceToHost, c->stream));
+    rc |= hip_ok(hipStreamSynchronize(c->stream));
+  }
+  dfree(d);
+  return rc;
+}
+
+int lbm_get_rho(lbm_ctx* c, float* rr, float* rb) {
+  if (!c) return LBM_ERR_ARG;
+  int rc = 0;
+  if (rr) rc |= hip_ok(hipMemcpyAsync(rr, c->rho_r, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+  if (rb) rc |= hip_ok(hipMemcpyAsync(rb, c->rho_b, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int lbm_get_moments(lbm_ctx* c, float* rr, float* rb, float* vel, float* ph) {
+  if (!c) return LBM_ERR_ARG;
+  float *d_rr = nullptr, *d_rb = nullptr, *d_v = nullptr, *d_ph = nullptr;
+  int rc = 0;
+  if (rr) rc |= dalloc(&d_rr, c->N);
+  if (rb) rc |= dalloc(&d_rb, c->N);
+  if (vel) rc |= dalloc(&d_v, 3 * c->N);
+  if (ph) rc |= dalloc(&d_ph, c->N);
+  if (!rc) {
+    rc |= lbm_launch_moments(c, d_rr, d_rb, d_v, d_ph);
+    if (rr) rc |= hip_ok(hipMemcpyAsync(rr, d_rr, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+    if (rb) rc |= hip_ok(hipMemcpyAsync(rb, d_rb, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+    if (vel) rc |= hip_ok(hipMemcpyAsync(vel, d_v, sizeof(float) * 3 * c->N, hipMemcpyDeviceToHost, c->stream));
+    if (ph) rc |= hip_ok(hipMemcpyAsync(ph, d_ph, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+    rc |= hip_ok(hipStreamSynchronize(c->stream));
+  }
+  dfree(d_rr); dfree(d_rb); dfree(d_v); dfree(d_ph);
+  return rc;
+}
+
+int lbm_get_flags(lbm_ctx* c, int8_t* out) {
+  if (!c || !out) return LBM_ERR_ARG;
+  int rc = hip_ok(hipMemcpyAsync(out, c->flags[c->cur], c->N, hipMemcpyDeviceToHost, c->stream));
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int lbm_get_particles(lbm_ctx* c, double* pos, double* force) {
+  if (!c || c->np <= 0) return LBM_ERR_ARG;
+  int rc = 0;
+  if (pos)
+    rc |= hip_ok(hipMemcpyAsync(pos, c->d_pos[c->cur], sizeof(double) * 13 * c->np, hipMemcpyDeviceToHost, c->stream));
+  if (force)
+    rc |= hip_ok(hipMemcpyAsync(force, c->d_force[1 - c->cur], sizeof(double) * 12 * c->np, hipMemcpyDeviceToHost,
+                                c->stream));
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int lbm_get_sums(lbm_ctx* c, double out[5]) {
+  if (!c || !out) return LBM_ERR_ARG;
+  double* d = nullptr;
+  int rc = dalloc(&d, 5);
+  if (!rc) {
+    rc |= lbm_launch_sums(c, d);
+    rc |= hip_ok(hipMemcpyAsync(out, d, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    rc |= hip_ok(hipStreamSynchronize(c->stream));
+  }
+  dfree(d);
+  return rc;
+}
+
+int lbm_status(lbm_ctx* c, int32_t* code, int32_t* step, char* msg, int32_t len) {
+  if (!c) return LBM_ERR_ARG;
+  lbm::DevStatus st{0, 0};
+  int rc = hip_ok(hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  int cd = c->host_err ? c->host_err : st.code;
+  int sp = c->host_err ? (int)c->host_err_step : st.step;
+  if (code) *code = cd;
+  if (step) *step = sp;
+  if (msg && len > 0) {
+    const char* m = "";
+    if (c->host_err) m = c->host_msg;
+    else if (st.code == LBM_ERR_RHO_RANGE) m = "init_rho_CG] Range error rho (kernels_fluid_CG.CUF:438)";
+    else if (st.code == LBM_ERR_BIG_FORCE) m = "nve_lf] Big force (kernels_fluid_PART.CUF:1532)";
+    else if (st.code == LBM_ERR_EXTREME_VEL) m = "nve_lf] Extreme vel (kernels_fluid_PART.CUF:1545)";
+    else if (st.code == LBM_ERR_OUT_OF_BOX) m = "nve] particle out of box (kernels_fluid_PART.CUF:1581)";
+    std::snprintf(msg, (size_t)len, "%s", m);
+  }
+  return rc;
+}
+
+int lbm_kernel_timing(lbm_ctx* c, int32_t on) {
+  if (!c) return LBM_ERR_ARG;
+  if (on && !c->ev_pairs) {
+    const int cap = 4096;
+    c->ev_pairs = new (std::nothrow) hipEvent_t[2 * cap];
+    if (!c->ev_pairs) return LBM_ERR_ALLOC;
+    for (int e = 0; e < 2 * cap; ++e)
+      if (hipEventCreate(&c->ev_pairs[e]) != hipSuccess) {
+        for (int q = 0; q < e; ++q) (void)hipEventDestroy(c->ev_pairs[q]);
+        delete[] c->ev_pairs;
+        c->ev_pairs = nullptr;
+        return LBM_ERR_HIP;
+      }
+    c->ev_cap = cap;
+  }
+  c->timing = on != 0;
+  c->ev_used = 0;
+  c->t_total_ms = 0.0;
+  c->t_launches = 0;
+  return LBM_OK;
+}
+
+}  /* extern "C" */
+
+/* Events are recorded on the library stream around each collide-stream
+ * launch; the elapsed times are summed only when asked for, so timing adds
+ * no host synchronisation to the step loop. */
+void lbm_timing_begin(lbm_ctx* c) {
+  if (!c->timing) return;
+  if (c->ev_used == c->ev_cap) {   /* pool full: fold the recorded pairs in */
+    int64_t n;
+    double ms;
+    lbm_get_kernel_timing(c, &ms, &n);
+  }
+  (void)hipEventRecord(c->ev_pairs[2 * c->ev_used], c->stream);
+}
+
+void lbm_timing_end(lbm_ctx* c) {
+  if (!c->timing) return;
+  (void)hipEventRecord(c->ev_pairs[2 * c->ev_used + 1], c->stream);
+  c->ev_used += 1;
+}
+
+extern "C" {
+
+int lbm_get_kernel_timing(lbm_ctx* c, double* total_ms, int64_t* launches) {
+  if (!c) return LBM_ERR_ARG;
+  if (c->ev_used > 0) {
+    if (hipEventSynchronize(c->ev_pairs[2 * c->ev_used - 1]) != hipSuccess) return LBM_ERR_HIP;
+    for (int e = 0; e < c->ev_used; ++e) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, c->ev_pairs[2 * e], c->ev_pairs[2 * e + 1]) != hipSuccess) return LBM_ERR_HIP;
+      c->t_total_ms += ms;
+    }
+    c->t_launches += c->ev_used;
+    c->ev_used = 0;
+  }
+  if (total_ms) *total_ms = c->t_total_ms;
+  if (launches) *launches = c->t_launches;
+  return LBM_OK;
+}
+
+int lbm_kernel_info(lbm_ctx* c, char* name, int32_t len, double* bytes_per_lu) {
+  if (!c) return LBM_ERR_ARG;
+  const char* st = c->storage == LBM_STORE_F32 ? "f32" : (c->storage == LBM_STORE_F16 ? "f16" : "bf16");
+  if (name && len > 0) std::snprintf(name, (size_t)len, "k_collide_push<%s,%s>", st, c->np > 0 ? "part" : "fluid");
+  if (bytes_per_lu) *bytes_per_lu = 2.0 * 19.0 * 2.0 * (double)c->esize + (c->np > 0 ? 1.0 : 0.0);
+  return LBM_OK;
+}
+
+}  /* extern "C" */

@@ -1,0 +1,175 @@
+/*
+ * lbm_common.h -- lattice constants, storage traits and the internal context
+ * shared by the HIP translation units of liblbcuda_amd.
+ *
+ * Lattice tables follow dimensions_m.CUF:201-300 (D3Q19 order, Leclaire B_l
+ * with xi = 0.5, D3Q27 weights for the phase-field gradient).  All constants
+ * are formed in float the way the reference's single-precision parameters are.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/lbcuda_amd.h"
+
+namespace lbm {
+
+/* node types, dimensions_m.CUF:305-309 */
+constexpr uint8_t FL_WALL = 0, FL_FLUID = 1, FL_LIST = 2, FL_DEAD = 4, FL_CM = 5;
+
+/* D3Q19 velocity set in the reference order (dimensions_m.CUF:212-219) */
+__host__ __device__ constexpr int EX(int l) {
+  return l == 1 || l == 7 || l == 10 || l == 11 || l == 14 ? 1
+       : (l == 2 || l == 8 || l == 9 || l == 12 || l == 13 ? -1 : 0);
+}
+__host__ __device__ constexpr int EY(int l) {
+  return l == 3 || l == 7 || l == 9 || l == 15 || l == 18 ? 1
+       : (l == 4 || l == 8 || l == 10 || l == 16 || l == 17 ? -1 : 0);
+}
+__host__ __device__ constexpr int EZ(int l) {
+  return l == 5 || l == 11 || l == 13 || l == 15 || l == 17 ? 1
+       : (l == 6 || l == 12 || l == 14 || l == 16 || l == 18 ? -1 : 0);
+}
+__host__ __device__ constexpr int OPP(int l) { return l == 0 ? 0 : (l % 2 == 1 ? l + 1 : l - 1); }
+
+/* weights p(l) and recolouring factors, dimensions_m.CUF:206-209, main.CUF:1102 */
+constexpr float CSSQ = 1.0f / 3.0f;
+constexpr float W0 = 1.0f / 3.0f, W1 = 1.0f / 18.0f, W2 = 1.0f / 36.0f;
+__host__ __device__ constexpr float W(int l) { return l == 0 ? W0 : (l < 7 ? W1 : W2); }
+/* Leclaire B_l, dimensions_m.CUF:222-228 */
+constexpr float BXI = 0.5f;
+constexpr float B0 = -(2.0f + 2.0f * BXI) / (3.0f * BXI + 12.0f);
+constexpr float B1 = BXI / (6.0f * BXI + 24.0f);
+constexpr float B2 = 1.0f / (6.0f * BXI + 24.0f);
+__host__ __device__ constexpr float BL(int l) { return l == 0 ? B0 : (l < 7 ? B1 : B2); }
+/* 1/|e_l| (rec_fact_d = |e_l|, main.CUF:1102; l = 0 gives cos(phi) = 0) */
+constexpr float INV_SQRT2 = 0.70710678118654752440f;
+__host__ __device__ constexpr float INV_REC(int l) { return l == 0 ? 0.0f : (l < 7 ? 1.0f : INV_SQRT2); }
+
+/* D3Q27 gradient weights ad3q27 = pd3q27/cssq, dimensions_m.CUF:283-300 */
+constexpr float P1D = (2.0f / 3.0f) * (2.0f / 3.0f) * (1.0f / 6.0f);
+constexpr float P2D = (1.0f / 6.0f) * (1.0f / 6.0f) * (2.0f / 3.0f);
+constexpr float P3D = (1.0f / 6.0f) * (1.0f / 6.0f) * (1.0f / 6.0f);
+constexpr float P0D = (2.0f / 3.0f) * (2.0f / 3.0f) * (2.0f / 3.0f);
+constexpr float A1D = P1D / CSSQ, A2D = P2D / CSSQ, A3D = P3D / CSSQ;
+
+/* pref_bouzidi = 2/cssq, dimensions_m.CUF:99 */
+constexpr float BUZ = 2.0f / CSSQ;
+
+/* ------------------------------------------------------------------ */
+/* storage traits: populations are kept either as float or as the shifted
+ * value f - w_l*rho0 in a 16-bit type; arithmetic is always float. */
+template <class S> struct Store;
+template <> struct Store<float> {
+  __device__ static float ld(const float* p, size_t i, float) { return p[i]; }
+  __device__ static void st(float* p, size_t i, float v, float) { p[i] = v; }
+};
+template <> struct Store<__half> {
+  __device__ static float ld(const __half* p, size_t i, float s) { return __half2float(p[i]) + s; }
+  __device__ static void st(__half* p, size_t i, float v, float s) { p[i] = __float2half_rn(v - s); }
+};
+template <> struct Store<__hip_bfloat16> {
+  __device__ static float ld(const __hip_bfloat16* p, size_t i, float s) { return __bfloat162float(p[i]) + s; }
+  __device__ static void st(__hip_bfloat16* p, size_t i, float v, float s) { p[i] = __float2bfloat16(v - s); }
+};
+
+/* device status word: first error wins */
+struct DevStatus {
+  int code;
+  int step;
+};
+__device__ inline void raise_error(DevStatus* st, int code, int step) {
+  if (atomicCAS(&st->code, 0, code) == 0) atomicExch(&st->step, step);
+}
+
+/* periodic wrap of a coordinate that is at most one box length off */
+__host__ __device__ inline int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+/* kernel arguments of the fluid path */
+struct FluidArgs {
+  int nx, ny, nz;
+  size_t N;              /* nx*ny*nz                                          */
+  const void* src;       /* state buffer (post-streaming), 2 colours x 19 x N */
+  void* dst;             /* next state buffer                                  */
+  const uint8_t* flags;  /* node types of this step                            */
+  float* rho_r;
+  float* rho_b;
+  float* phi;            /* phase field (rhoR-rhoB)/(rhoR+rhoB) per node       */
+  float inv_visc_r, inv_visc_b;
+  float sigma, beta;
+  float rho0;            /* storage shift                                      */
+  float min_rho, max_rho;
+  DevStatus* status;
+  int step;
+};
+
+}  // namespace lbm
+
+/* ---------------------------------------------------------------------- */
+/* internal context (host side) */
+struct lbm_ctx {
+  lbm_config cfg;
+  int nx, ny, nz;
+  size_t N;
+  hipStream_t stream;
+  int storage;
+  size_t esize;           /* bytes per stored population                      */
+  void* pop[2];           /* two time levels, each 2 x 19 x N                  */
+  float* rho_r;
+  float* rho_b;
+  float* phi;
+  uint8_t* flags[2];      /* flags of the two time levels                      */
+  lbm::DevStatus* d_status;
+  int cur;                /* index of the state buffer / current time level    */
+  long step;
+  bool initialised;
+  int host_err;
+  long host_err_step;
+  char host_msg[160];
+  /* particles */
+  int np;
+  double* d_pos[2];       /* (13, np) per time level                           */
+  double* d_force[2];     /* (12, np) per time level                           */
+  double* d_myf;          /* (3, np) FP64 sums: bounce-back force              */
+  double* d_myf2;         /* (3, np) fill/erase + pair forces                   */
+  double* d_myt;
+  double* d_myt2;
+  int* d_stamp[2];        /* (3, np) centres stamped into flags[b]             */
+  uint32_t* d_key[2];     /* (N) owner key per time level                      */
+  int* d_claim;           /* (N) make_fluid writer resolution                  */
+  int8_t* d_tmpl;         /* (4, ntmpl) template offsets + value               */
+  int ntmpl;
+  int8_t* d_ball;         /* (4, nball) offsets with r^2 <= rmax^2; [3]=1 if <   */
+  int nball;
+  int rmax;
+  /* timing: one event pair per collide-stream launch, read back lazily */
+  hipEvent_t* ev_pairs;   /* 2 * ev_cap events                                 */
+  int ev_cap;
+  int ev_used;
+  bool timing;
+  double t_total_ms;
+  long t_launches;
+};
+
+/* records the start/stop events around one timed launch (lbm_api.cpp) */
+void lbm_timing_begin(lbm_ctx* c);
+void lbm_timing_end(lbm_ctx* c);
+
+/* launchers implemented in lbm_fluid.hip / lbm_particles.hip */
+int lbm_launch_density(lbm_ctx* c);
+int lbm_launch_collide(lbm_ctx* c);
+int lbm_launch_init_eq(lbm_ctx* c, int type, const float u0[3], const float* d_vel);
+int lbm_launch_moments(lbm_ctx* c, float* d_rr, float* d_rb, float* d_vel, float* d_phase);
+int lbm_launch_sums(lbm_ctx* c, double* d_out);
+int lbm_launch_unpack_pops(lbm_ctx* c, int color, float* d_out);
+int lbm_launch_pack_pops(lbm_ctx* c, const float* d_pr, const float* d_pb);
+int lbm_particles_step_pre(lbm_ctx* c);   /* flags, erase, (density), fill, wetting */
+int lbm_particles_flags(lbm_ctx* c, int level);
+int lbm_particles_erase(lbm_ctx* c);
+int lbm_particles_fill(lbm_ctx* c);
+int lbm_particles_wetting(lbm_ctx* c);
+int lbm_particles_bounce(lbm_ctx* c);
+int lbm_particles_integrate(lbm_ctx* c);

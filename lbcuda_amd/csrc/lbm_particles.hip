@@ -1,0 +1,780 @@
+/*
+ * lbm_particles.hip -- colloid coupling kernels for gfx950.
+ *
+ * The reference runs every particle kernel over the whole grid, one thread per
+ * node, scanning a per-tile particle list (kernels_fluid_PART.CUF:85-180,
+ * 916-1040 with the lists of :2315-2451).  Here each kernel is
+ * particle-centric: one workgroup per particle walks the few hundred nodes
+ * around its centre (template or distance ball), so the work scales with the
+ * particle surface, not with the box.  Force and torque sums are reduced in
+ * FP64 inside the workgroup and written once per particle, so the result does
+ * not depend on scheduling.
+ *
+ * Node ownership: the reference gives a node to the first particle of the
+ * tile list that covers it (setmyfluid, PART:230-264).  Here every particle
+ * stamps key = (index<<3 | type) with atomicMin, so the lowest index wins.
+ */
+#include "lbm_common.h"
+
+#include <math.h>
+
+using namespace lbm;
+
+namespace {
+
+constexpr int TPB = 256;
+
+/* D3Q27 links in the reference order (dimensions_m.CUF:275-282) */
+__constant__ int8_t C_E27[27][3] = {
+    {0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},  {0, 0, 1},  {0, 0, -1},
+    {1, 1, 0},   {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}, {1, 0, 1},   {-1, 0, -1}, {-1, 0, 1},
+    {1, 0, -1},  {0, 1, 1},   {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {1, 1, 1},  {-1, -1, -1},
+    {1, 1, -1},  {-1, -1, 1}, {-1, 1, 1}, {1, -1, -1}, {1, -1, 1}, {-1, 1, -1}};
+/* D3Q27 weights pd3q27 */
+__device__ __forceinline__ float PD27(int l) { return l == 0 ? P0D : (l < 7 ? P1D : (l < 19 ? P2D : P3D)); }
+
+/* second-belt offsets, main.CUF:1145-1161 (k, j, i loop order) */
+__constant__ int8_t C_BELT2[98][3];
+
+struct PartArgs {
+  int nx, ny, nz;
+  size_t N;
+  int np;
+  int cur;                 /* current level (flip)             */
+  const double* pos_cur;   /* (13, np)                          */
+  const double* pos_old;   /* (13, np), the other level         */
+  double* pos_new;         /* written by the integrator         */
+  double* f_cur;           /* forceAtoms(12, np) current level   */
+  double* f_old;           /* forceAtoms other level             */
+  double* myf;
+  double* myf2;
+  double* myt;
+  double* myt2;
+  uint8_t* fl_cur;
+  const uint8_t* fl_old;
+  uint32_t* key_cur;
+  int* stamp_cur;          /* (3, np)                            */
+  int* claim;
+  const int8_t* tmpl;      /* (4, ntmpl)                         */
+  int ntmpl;
+  const int8_t* ball;      /* (4, nball), [3] = 1 if r^2 < rmax^2 */
+  int nball;
+  float* rho_r;
+  float* rho_b;
+  float* phi;
+  void* pop_src;           /* state buffer of this step          */
+  void* pop_dst;           /* next buffer (after collide)        */
+  float rho0;
+  float radius;
+  int lrotate;
+  int wetting;
+  DevStatus* status;
+  int step;
+};
+
+__device__ __forceinline__ long nint(double x) { return lround(x); }   /* Fortran NINT */
+
+__device__ __forceinline__ size_t node_at(const PartArgs& a, int cx, int cy, int cz, int dx, int dy, int dz,
+                                          int* gx = nullptr, int* gy = nullptr, int* gz = nullptr) {
+  /* centres are 1-based lattice coordinates, nodes 0-based */
+  int x = cx - 1 + dx, y = cy - 1 + dy, z = cz - 1 + dz;
+  x = ((x % a.nx) + a.nx) % a.nx;
+  y = ((y % a.ny) + a.ny) % a.ny;
+  z = ((z % a.nz) + a.nz) % a.nz;
+  if (gx) { *gx = x; *gy = y; *gz = z; }
+  return (size_t)x + (size_t)a.nx * ((size_t)y + (size_t)a.ny * z);
+}
+
+__device__ __forceinline__ size_t wrap_node(const PartArgs& a, int x, int y, int z) {
+  x = wrapc(x, a.nx); y = wrapc(y, a.ny); z = wrapc(z, a.nz);
+  return (size_t)x + (size_t)a.nx * ((size_t)y + (size_t)a.ny * z);
+}
+
+__device__ void centre(const double* pos, int p, int c[3]) {
+  c[0] = (int)nint(pos[13 * p + 0]);
+  c[1] = (int)nint(pos[13 * p + 1]);
+  c[2] = (int)nint(pos[13 * p + 2]);
+}
+
+__device__ bool moved(const PartArgs& a, int p) {   /* particle_moved, PART:363-380 */
+  int c1[3], c2[3];
+  centre(a.pos_cur, p, c1);
+  centre(a.pos_old, p, c2);
+  return c1[0] != c2[0] || c1[1] != c2[1] || c1[2] != c2[2];
+}
+
+/* workgroup FP64 reduction of 6 values; result valid in thread 0 */
+__device__ void block_reduce6(double v[6]) {
+  __shared__ double sh[6][TPB / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    double x = v[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) sh[q][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < TPB / 64; ++w) s += sh[q][w];
+      v[q] = s;
+    }
+  __syncthreads();
+}
+
+__device__ void cross3(const double a[3], const double b[3], double o[3]) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* lever arm (R / |r|) r from the particle to node (gx,gy,gz) (0-based);
+ * minimum image only when minimg (del_fluid/make_fluid skip it for
+ * spheredead nodes, PART:537-545, 722-727). */
+__device__ void lever(const PartArgs& a, const double* pos, int p, int gx, int gy, int gz, bool minimg,
+                      double r[3]) {
+  r[0] = (double)(gx + 1) - pos[13 * p + 0];
+  r[1] = (double)(gy + 1) - pos[13 * p + 1];
+  r[2] = (double)(gz + 1) - pos[13 * p + 2];
+  if (minimg) {
+    r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
+    r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
+    r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
+  }
+  double s = (double)a.radius / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  r[0] *= s; r[1] *= s; r[2] *= s;
+}
+
+/* Hamilton product o = a b (qmult, PART:1854-1860) */
+__device__ void qmul(const double a[4], const double b[4], double o[4]) {
+  o[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  o[1] = a[0] * b[1] + b[0] * a[1] + (a[2] * b[3] - a[3] * b[2]);
+  o[2] = a[0] * b[2] + b[0] * a[2] + (a[3] * b[1] - a[1] * b[3]);
+  o[3] = a[0] * b[3] + b[0] * a[3] + (a[1] * b[2] - a[2] * b[1]);
+}
+
+/* lab-frame angular velocity q (0,omega_body) q*, PART:1900-1912 */
+__device__ void omega_lab(const double* pos, int p, double o[3]) {
+  const double q[4] = {pos[13 * p + 9], pos[13 * p + 10], pos[13 * p + 11], pos[13 * p + 12]};
+  const double qc[4] = {q[0], -q[1], -q[2], -q[3]};
+  const double w[4] = {0.0, pos[13 * p + 6], pos[13 * p + 7], pos[13 * p + 8]};
+  double t[4], r[4];
+  qmul(w, qc, t);
+  qmul(q, t, r);
+  o[0] = r[1];
+  o[1] = r[2];
+  o[2] = r[3];
+}
+
+template <class S>
+__device__ __forceinline__ float ldp(const void* p, size_t i, int l, float rho0) {
+  return Store<S>::ld((const S*)p, i, W(l) * rho0);
+}
+template <class S>
+__device__ __forceinline__ void stp(void* p, size_t i, int l, float v, float rho0) {
+  Store<S>::st((S*)p, i, v, W(l) * rho0);
+}
+
+/* atomic subtraction on a stored population (used by the moving-wall term) */
+__device__ __forceinline__ void atomic_sub_pop(float* p, size_t i, float v, float) { atomicAdd(p + i, -v); }
+template <class S>
+__device__ void atomic_sub_pop16(S* p, size_t i, float v, float shift) {
+  /* 16-bit value inside an aligned 32-bit word: CAS loop */
+  uintptr_t addr = (uintptr_t)(p + i);
+  unsigned int* w = (unsigned int*)(addr & ~(uintptr_t)3);
+  const int hi = (addr & 2) ? 1 : 0;
+  unsigned int old = *w, assumed;
+  do {
+    assumed = old;
+    unsigned short h = hi ? (unsigned short)(assumed >> 16) : (unsigned short)(assumed & 0xffffu);
+    S sv;
+    __builtin_memcpy(&sv, &h, 2);
+    float f = Store<S>::ld(&sv, 0, shift) - v;
+    Store<S>::st(&sv, 0, f, shift);
+    __builtin_memcpy(&h, &sv, 2);
+    unsigned int nw = hi ? ((assumed & 0x0000ffffu) | ((unsigned int)h << 16)) : ((assumed & 0xffff0000u) | h);
+    old = atomicCAS(w, assumed, nw);
+  } while (old != assumed);
+}
+template <class S>
+__device__ __forceinline__ void atomic_sub(void* p, size_t i, int l, float v, float rho0) {
+  if constexpr (sizeof(S) == 4) atomic_sub_pop((float*)p, i, v, 0.0f);
+  else atomic_sub_pop16<S>((S*)p, i, v, W(l) * rho0);
+}
+
+/* ---------------- flags ------------------------------------------------ */
+__global__ __launch_bounds__(TPB) void k_unstamp(PartArgs a) {
+  const int p = blockIdx.x;
+  const int cx = a.stamp_cur[3 * p], cy = a.stamp_cur[3 * p + 1], cz = a.stamp_cur[3 * p + 2];
+  if (cx == INT32_MIN) return;
+  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+    size_t n = node_at(a, cx, cy, cz, a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
+    a.fl_cur[n] = FL_FLUID;
+    a.key_cur[n] = 0xffffffffu;
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_stamp(PartArgs a) {
+  const int p = blockIdx.x;
+  int c[3];
+  centre(a.pos_cur, p, c);
+  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+    size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
+    atomicMin(&a.key_cur[n], ((uint32_t)p << 3) | (uint32_t)a.tmpl[4 * t + 3]);
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_resolve(PartArgs a) {
+  const int p = blockIdx.x;
+  int c[3];
+  centre(a.pos_cur, p, c);
+  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+    size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
+    uint32_t k = a.key_cur[n];
+    if ((k >> 3) == (uint32_t)p) a.fl_cur[n] = (uint8_t)(k & 7u);
+  }
+  if (threadIdx.x == 0) {
+    a.stamp_cur[3 * p] = c[0];
+    a.stamp_cur[3 * p + 1] = c[1];
+    a.stamp_cur[3 * p + 2] = c[2];
+  }
+}
+
+/* ---------------- erase: fluid -> solid (del_fluid, PART:383-561) -------- */
+template <class S>
+__global__ __launch_bounds__(TPB) void k_erase(PartArgs a) {
+  const int p = blockIdx.x;
+  if (!moved(a, p)) return;
+  int co[3];
+  centre(a.pos_old, p, co);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+    int gx, gy, gz;
+    size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
+    const uint8_t fnew = a.fl_cur[n];
+    if (a.fl_old[n] != FL_FLUID || !(fnew == FL_LIST || fnew == FL_DEAD)) continue;
+    const float rR = a.rho_r[n], rB = a.rho_b[n];
+    const float inv = 1.0f / (rR + rB);
+    float fr[19], fb[19];
+#pragma unroll
+    for (int l = 0; l < 19; ++l) {
+      fr[l] = ldp<S>(a.pop_src, (size_t)l * a.N + n, l, a.rho0);
+      fb[l] = ldp<S>(a.pop_src, (size_t)(19 + l) * a.N + n, l, a.rho0);
+    }
+    const float u = inv * (fr[1] - fr[2] + fr[7] - fr[8] - fr[9] + fr[10] + fr[11] - fr[12] - fr[13] + fr[14]) +
+                    inv * (fb[1] - fb[2] + fb[7] - fb[8] - fb[9] + fb[10] + fb[11] - fb[12] - fb[13] + fb[14]);
+    const float v = inv * (fr[3] - fr[4] + fr[7] - fr[8] + fr[9] - fr[10] + fr[15] - fr[16] - fr[17] + fr[18]) +
+                    inv * (fb[3] - fb[4] + fb[7] - fb[8] + fb[9] - fb[10] + fb[15] - fb[16] - fb[17] + fb[18]);
+    const float w = inv * (fr[5] - fr[6] + fr[11] - fr[12] + fr[13] - fr[14] + fr[15] - fr[16] + fr[17] - fr[18]) +
+                    inv * (fb[5] - fb[6] + fb[11] - fb[12] + fb[13] - fb[14] + fb[15] - fb[16] + fb[17] - fb[18]);
+    double ft[3] = {(double)((rR + rB) * u), (double)((rR + rB) * v), (double)((rR + rB) * w)};
+    acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
+    if (a.lrotate) {
+      double r[3], tq[3];
+      lever(a, a.pos_cur, p, gx, gy, gz, fnew == FL_LIST, r);
+      cross3(r, ft, tq);
+      acc[3] += tq[0]; acc[4] += tq[1]; acc[5] += tq[2];
+    }
+  }
+  block_reduce6(acc);
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) a.myf2[3 * p + d] += acc[d];
+    if (a.lrotate)
+      for (int d = 0; d < 3; ++d) a.myt2[3 * p + d] += acc[3 + d];
+  }
+}
+
+/* ---------------- fill: solid -> fluid (make_fluid, PART:563-747) -------- */
+__device__ bool fill_candidate(const PartArgs& a, size_t n) {
+  const uint8_t fo = a.fl_old[n];
+  return (fo == FL_LIST || fo == FL_DEAD) && a.fl_cur[n] == FL_FLUID;
+}
+
+__global__ __launch_bounds__(TPB) void k_fill_claim(PartArgs a) {
+  const int p = blockIdx.x;
+  if (!moved(a, p)) return;
+  int co[3];
+  centre(a.pos_old, p, co);
+  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+    size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2]);
+    if (fill_candidate(a, n)) atomicMax(&a.claim[n], p);
+  }
+}
+
+template <class S>
+__global__ __launch_bounds__(TPB) void k_fill_apply(PartArgs a) {
+  const int p = blockIdx.x;
+  if (!moved(a, p)) return;
+  int co[3];
+  centre(a.pos_old, p, co);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const float vx = (float)a.pos_cur[13 * p + 3], vy = (float)a.pos_cur[13 * p + 4], vz = (float)a.pos_cur[13 * p + 5];
+  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+    int gx, gy, gz;
+    size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
+    if (!fill_candidate(a, n)) continue;
+    const uint8_t fo = a.fl_old[n];
+    /* compute_onebelt_density_twofluids, PART:783-813 */
+    float Rs = 0.0f, Bs = 0.0f;
+    int D = 0;
+    for (int l = 1; l < 27; ++l) {
+      size_t m = wrap_node(a, gx + C_E27[l][0], gy + C_E27[l][1], gz + C_E27[l][2]);
+      if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
+        Rs = Rs + a.rho_r[m];
+        Bs = Bs + a.rho_b[m];
+        D = D + 1;
+      }
+    }
+    if (D == 0) {
+      /* compute_secbelt_density_twofluids, PART:815-852 */
+      for (int l = 0; l < 98; ++l) {
+        size_t m = wrap_node(a, gx + C_BELT2[l][0], gy + C_BELT2[l][1], gz + C_BELT2[l][2]);
+        if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
+          Rs = Rs + a.rho_r[m];
+          Bs = Bs + a.rho_b[m];
+          D = D + 1;
+        }
+      }
+    }
+    if (D != 0) { Rs = Rs / (float)D; Bs = Bs / (float)D; }
+    /* the highest-index particle writes the node, as the last list entry does */
+    if (a.claim[n] == p) {
+      a.rho_r[n] = Rs;
+      a.rho_b[n] = Bs;
+      a.phi[n] = (Rs - Bs) / (Rs + Bs);
+      const float usq = (0.5f / CSSQ) * (vx * vx + vy * vy + vz * vz);
+#pragma unroll
+      for (int l = 0; l < 19; ++l) {
+        const float uv = (1.0f / CSSQ) * (vx * (float)EX(l) + vy * (float)EY(l) + vz * (float)EZ(l));
+        const float brk = 1.0f + uv + 0.5f * (uv * uv) - usq;
+        stp<S>(a.pop_src, (size_t)l * a.N + n, l, Rs * W(l) * brk, a.rho0);
+        stp<S>(a.pop_src, (size_t)(19 + l) * a.N + n, l, Bs * W(l) * brk, a.rho0);
+      }
+    }
+    double ft[3] = {(double)(-(Rs + Bs) * vx), (double)(-(Rs + Bs) * vy), (double)(-(Rs + Bs) * vz)};
+    acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
+    if (a.lrotate) {
+      double r[3], tq[3];
+      lever(a, a.pos_cur, p, gx, gy, gz, fo == FL_LIST, r);
+      cross3(r, ft, tq);
+      acc[3] += tq[0]; acc[4] += tq[1]; acc[5] += tq[2];
+    }
+  }
+  block_reduce6(acc);
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) a.myf2[3 * p + d] += acc[d];
+    if (a.lrotate)
+      for (int d = 0; d < 3; ++d) a.myt2[3 * p + d] += acc[3 + d];
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_fill_release(PartArgs a) {
+  const int p = blockIdx.x;
+  if (!moved(a, p)) return;
+  int co[3];
+  centre(a.pos_old, p, co);
+  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+    size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2]);
+    a.claim[n] = -1;
+  }
+}
+
+/* ---------------- wetting of surface nodes -------------------------------- */
+__global__ __launch_bounds__(TPB) void k_wetting(PartArgs a) {
+  const int p = blockIdx.x;
+  int c[3];
+  centre(a.pos_cur, p, c);
+  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+    if (a.tmpl[4 * t + 3] != FL_LIST) continue;
+    int gx, gy, gz;
+    size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2], &gx, &gy, &gz);
+    if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) continue;
+    float rr, rb;
+    if (a.wetting == LBM_WET_JANUS) {
+      /* compute_virtual_rho, PART:299-352: sign of r.(q x q*) */
+      double r[3] = {(double)(gx + 1) - a.pos_cur[13 * p], (double)(gy + 1) - a.pos_cur[13 * p + 1],
+                     (double)(gz + 1) - a.pos_cur[13 * p + 2]};
+      r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
+      r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
+      r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
+      const double q0 = a.pos_cur[13 * p + 9], q1 = a.pos_cur[13 * p + 10], q2 = a.pos_cur[13 * p + 11],
+                   q3 = a.pos_cur[13 * p + 12];
+      /* body x axis in the lab frame: first column of the rotation matrix */
+      const double ax = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+      const double ay = 2.0 * (q1 * q2 + q0 * q3);
+      const double az = 2.0 * (q1 * q3 - q0 * q2);
+      const double ac = ax * r[0] + ay * r[1] + az * r[2];
+      if (ac < 0.0) { rr = 0.0f; rb = 1.0f; } else { rr = 1.0f; rb = 0.0f; }
+    } else {
+      /* compute_densities_wall + compute_onebelt_density_2fl_weights, PART:267-297, 864-894 */
+      float Rs = 0.0f, Bs = 0.0f, Ds = 0.0f;
+      for (int l = 1; l < 27; ++l) {
+        size_t m = wrap_node(a, gx + C_E27[l][0], gy + C_E27[l][1], gz + C_E27[l][2]);
+        if (a.fl_cur[m] == FL_FLUID) {
+          Rs = Rs + PD27(l) * a.rho_r[m];
+          Bs = Bs + PD27(l) * a.rho_b[m];
+          Ds = Ds + PD27(l);
+        }
+      }
+      if (Ds != 0.0f) {
+        rr = Rs / Ds;
+        rb = Bs / Ds;
+      } else {
+        float R2 = 0.0f, B2 = 0.0f;
+        int D = 0;
+        for (int l = 0; l < 98; ++l) {
+          size_t m = wrap_node(a, gx + C_BELT2[l][0], gy + C_BELT2[l][1], gz + C_BELT2[l][2]);
+          if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
+            R2 = R2 + a.rho_r[m];
+            B2 = B2 + a.rho_b[m];
+            D = D + 1;
+          }
+        }
+        if (D == 0) continue;   /* fix_onebelt_density_twofluids is empty */
+        rr = R2 / (float)D;
+        rb = B2 / (float)D;
+      }
+    }
+    a.rho_r[n] = rr;
+    a.rho_b[n] = rb;
+    a.phi[n] = (rr - rb) / (rr + rb);
+  }
+}
+
+/* ---------------- bounce-back on surface links ---------------------------
+ * node_to_particle_bounce_back_bc2 (PART:1874-2113) and the moving-wall part
+ * of particle_to_node_bounce_back_bc2_phase2 (PART:2201-2312), after the
+ * collide-stream kernel: the post-collision value f_hi(ii) that the fluid
+ * node ii pushed into the surface node s sits in dst(s, hi); the bounce-back
+ * copy the fluid node made for itself sits in dst(ii, lo) and receives the
+ * wall term here.  Every particle whose centre is within r^2 < rmax^2 of a
+ * surface node acts on it, as in partBB_CG / partBB_CG_p2n2. */
+template <class S>
+__global__ __launch_bounds__(TPB) void k_bounce(PartArgs a) {
+  const int p = blockIdx.x;
+  int c[3];
+  centre(a.pos_cur, p, c);
+  const double vs[3] = {a.pos_cur[13 * p + 3], a.pos_cur[13 * p + 4], a.pos_cur[13 * p + 5]};
+  double ot[3] = {0, 0, 0};
+  if (a.lrotate) omega_lab(a.pos_cur, p, ot);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+    if (!a.ball[4 * t + 3]) continue;   /* r^2 < sphereMax */
+    int gx, gy, gz;
+    const size_t s = node_at(a, c[0], c[1], c[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
+    if (a.fl_cur[s] != FL_LIST) continue;
+    double rv[3] = {0, 0, 0};
+    if (a.lrotate) lever(a, a.pos_cur, p, gx, gy, gz, true, rv);
+#pragma unroll
+    for (int il = 1; il <= 9; ++il) {
+      const int lo = 2 * il - 1, hi = 2 * il;
+      const size_t ii = wrap_node(a, gx + EX(lo), gy + EY(lo), gz + EZ(lo));
+      const size_t io = wrap_node(a, gx + EX(hi), gy + EY(hi), gz + EZ(hi));
+      const bool fi = a.fl_cur[ii] == FL_FLUID, fo = a.fl_cur[io] == FL_FLUID;
+      if (fi == fo) continue;
+      const int lin = fi ? lo : hi;      /* from s toward the fluid node   */
+      const int lout = fi ? hi : lo;     /* from the fluid node toward s   */
+      const size_t nf = fi ? ii : io;
+      double v[3] = {vs[0], vs[1], vs[2]};
+      double rt[3] = {0, 0, 0};
+      if (a.lrotate) {
+        rt[0] = rv[0] + 0.5 * EX(lin);
+        rt[1] = rv[1] + 0.5 * EY(lin);
+        rt[2] = rv[2] + 0.5 * EZ(lin);
+        double cr[3];
+        cross3(ot, rt, cr);
+        v[0] += cr[0]; v[1] += cr[1]; v[2] += cr[2];
+      }
+      const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
+      const float wr = W(lout) * BUZ * a.rho_r[nf] * ev;
+      const float wb = W(lout) * BUZ * a.rho_b[nf] * ev;
+      const float fR = ldp<S>(a.pop_dst, (size_t)lout * a.N + s, lout, a.rho0);
+      const float fB = ldp<S>(a.pop_dst, (size_t)(19 + lout) * a.N + s, lout, a.rho0);
+      const double f2p = (double)((2.0f * fR - wr) + (2.0f * fB - wb));
+      double ft[3] = {f2p * EX(lout), f2p * EY(lout), f2p * EZ(lout)};
+      acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
+      if (a.lrotate) {
+        double tq[3];
+        cross3(rt, ft, tq);
+        acc[3] += tq[0] * abs(EX(lout));
+        acc[4] += tq[1] * abs(EY(lout));
+        acc[5] += tq[2] * abs(EZ(lout));
+      }
+      /* moving wall: f_lin at the fluid node -= 6 w rho_k (e.v) */
+      atomic_sub<S>(a.pop_dst, (size_t)lin * a.N + nf, lin, wr, a.rho0);
+      atomic_sub<S>(a.pop_dst, (size_t)(19 + lin) * a.N + nf, lin, wb, a.rho0);
+    }
+  }
+  block_reduce6(acc);
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) a.myf[3 * p + d] = acc[d];
+    if (a.lrotate)
+      for (int d = 0; d < 3; ++d) a.myt[3 * p + d] = acc[3 + d];
+  }
+}
+
+/* ---------------- pair forces (inter_force_particles, PART:1052-1068) ---- */
+struct PairParams {
+  double hz_k, hz_rmin, hz_rcap, gmin;
+  double rpar, lubfactor, rparcut, rparcap, mxrsqcut, visc;
+  int hertz, lub;
+};
+
+__global__ __launch_bounds__(TPB) void k_pairs(PartArgs a, PairParams q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  double f[3] = {0, 0, 0};
+  const double xi = a.pos_cur[13 * i], yi = a.pos_cur[13 * i + 1], zi = a.pos_cur[13 * i + 2];
+  for (int j = 0; j < a.np; ++j) {
+    if (j == i) continue;
+    /* addHertzian, PART:1385-1503, with the pair vector from i to j */
+    double xdf = a.pos_cur[13 * j] - xi, ydf = a.pos_cur[13 * j + 1] - yi, zdf = a.pos_cur[13 * j + 2] - zi;
+    xdf -= (double)a.nx * (double)nint(xdf / (double)a.nx);
+    ydf -= (double)a.ny * (double)nint(ydf / (double)a.ny);
+    zdf -= (double)a.nz * (double)nint(zdf / (double)a.nz);
+    const double rsq = xdf * xdf + ydf * ydf + zdf * zdf;
+    if (rsq > q.mxrsqcut) continue;
+    const double rrr = sqrt(rsq);
+    if (q.hertz && rrr <= q.hz_rmin) {
+      const double g = rrr <= q.hz_rcap ? q.gmin : 2.5 * q.hz_k * pow(q.hz_rmin - rrr, 1.5);
+      f[0] -= g * xdf / rrr; f[1] -= g * ydf / rrr; f[2] -= g * zdf / rrr;
+    }
+    if (q.lub && rrr <= q.rparcut) {
+      const double ux = xdf / rrr, uy = ydf / rrr, uz = zdf / rrr;
+      const double rr = rrr <= q.rparcap ? q.rparcap : rrr;
+      const double dotuv = ux * (a.pos_cur[13 * i + 3] - a.pos_cur[13 * j + 3]) +
+                           uy * (a.pos_cur[13 * i + 4] - a.pos_cur[13 * j + 4]) +
+                           uz * (a.pos_cur[13 * i + 5] - a.pos_cur[13 * j + 5]);
+      const double s = q.visc * q.lubfactor * dotuv * (1.0 / (rr - q.rpar) - 1.0);
+      f[0] -= s * ux; f[1] -= s * uy; f[2] -= s * uz;
+    }
+  }
+  for (int d = 0; d < 3; ++d) a.myf2[3 * i + d] += f[d];
+}
+
+/* ---------------- rigid-body update --------------------------------------
+ * time_step_force_particles -> addForce_particle_bounce_back -> nve_lf ->
+ * rot_midstep_impl_lf (PART:1098-1166, 1266-1382, 1506-1614, 1621-1827),
+ * in FP64. */
+struct IntParams {
+  double rmass, rotin, max_vel;
+  double ext_f[3], ext_t[3];
+};
+
+__global__ __launch_bounds__(TPB) void k_integrate(PartArgs a, IntParams q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  const double* P = a.pos_cur + 13 * i;
+  double* Pn = a.pos_new + 13 * i;
+  double* Fc = a.f_cur + 12 * i;
+  double* Fo = a.f_old + 12 * i;
+  /* addForce_particle_bounce_back: F(t) = F_other + (F_bb(t+1/2)+F_bb(t-1/2))/2 */
+  for (int d = 0; d < 3; ++d) {
+    Fc[3 + d] = a.myf[3 * i + d];
+    Fc[d] = a.myf2[3 * i + d] + (Fc[3 + d] + Fo[3 + d]) * 0.5;
+  }
+  if (a.lrotate)
+    for (int d = 0; d < 3; ++d) {
+      Fc[9 + d] = a.myt[3 * i + d];
+      Fc[6 + d] = a.myt2[3 * i + d] + (Fc[9 + d] + Fo[9 + d]) * 0.5;
+    }
+  /* nve_lf */
+  const double fx = Fc[0], fy = Fc[1], fz = Fc[2];
+  if (fabs(fx) * q.rmass > 0.5 || fabs(fy) * q.rmass > 0.5 || fabs(fz) * q.rmass > 0.5)
+    raise_error(a.status, LBM_ERR_BIG_FORCE, a.step);
+  const double bx = P[3] + q.rmass * fx, by = P[4] + q.rmass * fy, bz = P[5] + q.rmass * fz;
+  if (sqrt(bx * bx + by * by + bz * bz) > q.max_vel) raise_error(a.status, LBM_ERR_EXTREME_VEL, a.step);
+  double xn[3] = {P[0] + bx, P[1] + by, P[2] + bz};
+  const int nn[3] = {a.nx, a.ny, a.nz};
+  for (int d = 0; d < 3; ++d) {
+    /* pbc_images_centered about the box centre (n+1)/2, PART:2124-2151 */
+    const double cc = (double)(nn[d] + 1) * 0.5;
+    const double s = xn[d] - cc;
+    xn[d] = s - (double)nn[d] * (double)nint(s / (double)nn[d]) + cc;
+    if (xn[d] < 0.5 || xn[d] > (double)nn[d] + 0.5) raise_error(a.status, LBM_ERR_OUT_OF_BOX, a.step);
+  }
+  if (a.lrotate && q.rmass > 0.0) {
+    const double I = q.rotin;
+    double ox = P[6], oy = P[7], oz = P[8];
+    double q0 = P[9], q1 = P[10], q2 = P[11], q3 = P[12];
+    const double tqx = Fc[6], tqy = Fc[7], tqz = Fc[8];
+    /* quat_2_rotmat, PART:1830-1844 */
+    const double m1 = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3, m2 = 2.0 * (q1 * q2 - q0 * q3),
+                 m3 = 2.0 * (q1 * q3 + q0 * q2), m4 = 2.0 * (q1 * q2 + q0 * q3),
+                 m5 = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3, m6 = 2.0 * (q2 * q3 - q0 * q1),
+                 m7 = 2.0 * (q1 * q3 - q0 * q2), m8 = 2.0 * (q2 * q3 + q0 * q1),
+                 m9 = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+    double opx = ox, opy = oy, opz = oz;
+    const double trx = (tqx * m1 + tqy * m4 + tqz * m7) / I + (I - I) * opy * opz / I;
+    const double tr_y = (tqx * m2 + tqy * m5 + tqz * m8) / I + (I - I) * opz * opx / I;
+    const double trz = (tqx * m3 + tqy * m6 + tqz * m9) / I + (I - I) * opx * opy / I;
+    opx = ox + trx * 0.5; opy = oy + tr_y * 0.5; opz = oz + trz * 0.5;
+    double dq0 = (-q1 * opx - q2 * opy - q3 * opz) * 0.5;
+    double dq1 = (q0 * opx - q3 * opy + q2 * opz) * 0.5;
+    double dq2 = (q3 * opx + q0 * opy - q1 * opz) * 0.5;
+    double dq3 = (-q2 * opx + q1 * opy + q0 * opz) * 0.5;
+    double qn0 = q0 + dq0 * 0.5, qn1 = q1 + dq1 * 0.5, qn2 = q2 + dq2 * 0.5, qn3 = q3 + dq3 * 0.5;
+    double rn = 1.0 / sqrt(qn0 * qn0 + qn1 * qn1 + qn2 * qn2 + qn3 * qn3);
+    qn0 *= rn; qn1 *= rn; qn2 *= rn; qn3 *= rn;
+    ox += trx; oy += tr_y; oz += trz;
+    Pn[6] = ox; Pn[7] = oy; Pn[8] = oz;
+    opx = ox; opy = oy; opz = oz;
+    int itq = 0;
+    double eps = 1.0e9;
+    while (itq < 20 && eps > 1.0e-24) {
+      ++itq;
+      dq0 = (-qn1 * opx - qn2 * opy - qn3 * opz) * 0.5;
+      dq1 = (qn0 * opx - qn3 * opy + qn2 * opz) * 0.5;
+      dq2 = (qn3 * opx + qn0 * opy - qn1 * opz) * 0.5;
+      dq3 = (-qn2 * opx + qn1 * opy + qn0 * opz) * 0.5;
+      double a0 = qn0 + dq0 * 0.5, a1 = qn1 + dq1 * 0.5, a2 = qn2 + dq2 * 0.5, a3 = qn3 + dq3 * 0.5;
+      rn = 1.0 / sqrt(a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3);
+      a0 *= rn; a1 *= rn; a2 *= rn; a3 *= rn;
+      eps = (a0 - qn0) * (a0 - qn0) + (a1 - qn1) * (a1 - qn1) + (a2 - qn2) * (a2 - qn2) + (a3 - qn3) * (a3 - qn3);
+      qn0 = a0; qn1 = a1; qn2 = a2; qn3 = a3;
+    }
+    q0 += dq0; q1 += dq1; q2 += dq2; q3 += dq3;
+    rn = 1.0 / sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+    Pn[9] = q0 * rn; Pn[10] = q1 * rn; Pn[11] = q2 * rn; Pn[12] = q3 * rn;
+  } else {
+    for (int c = 6; c < 13; ++c) Pn[c] = P[c];
+  }
+  Pn[0] = xn[0]; Pn[1] = xn[1]; Pn[2] = xn[2];
+  Pn[3] = bx; Pn[4] = by; Pn[5] = bz;
+  /* reset accumulators, PART:1117-1145 */
+  for (int d = 0; d < 3; ++d) {
+    Fo[d] = q.ext_f[d];
+    Fo[3 + d] = 0.0;
+    a.myf[3 * i + d] = 0.0;
+    a.myf2[3 * i + d] = q.ext_f[d];
+  }
+  if (a.lrotate)
+    for (int d = 0; d < 3; ++d) {
+      Fo[6 + d] = q.ext_t[d];
+      Fo[9 + d] = 0.0;
+      a.myt[3 * i + d] = 0.0;
+      a.myt2[3 * i + d] = q.ext_t[d];
+    }
+}
+
+PartArgs make_pargs(lbm_ctx* c) {
+  PartArgs a;
+  a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
+  a.N = c->N;
+  a.np = c->np;
+  a.cur = c->cur;
+  a.pos_cur = c->d_pos[c->cur];
+  a.pos_old = c->d_pos[1 - c->cur];
+  a.pos_new = c->d_pos[1 - c->cur];
+  a.f_cur = c->d_force[c->cur];
+  a.f_old = c->d_force[1 - c->cur];
+  a.myf = c->d_myf; a.myf2 = c->d_myf2; a.myt = c->d_myt; a.myt2 = c->d_myt2;
+  a.fl_cur = c->flags[c->cur];
+  a.fl_old = c->flags[1 - c->cur];
+  a.key_cur = c->d_key[c->cur];
+  a.stamp_cur = c->d_stamp[c->cur];
+  a.claim = c->d_claim;
+  a.tmpl = c->d_tmpl; a.ntmpl = c->ntmpl;
+  a.ball = c->d_ball; a.nball = c->nball;
+  a.rho_r = c->rho_r; a.rho_b = c->rho_b; a.phi = c->phi;
+  a.pop_src = c->pop[c->cur];
+  a.pop_dst = c->pop[1 - c->cur];
+  a.rho0 = c->storage == LBM_STORE_F32 ? 0.0f : c->cfg.store_rho0;
+  a.radius = c->cfg.radius;
+  a.lrotate = c->cfg.lrotate;
+  a.wetting = c->cfg.wetting;
+  a.status = c->d_status;
+  a.step = (int)c->step;
+  return a;
+}
+
+int last_err() { return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP; }
+
+}  // namespace
+
+#define LBM_STORAGE_SWITCH(st, BODY)                                  \
+  switch (st) {                                                       \
+    case LBM_STORE_F32: { using S = float; BODY; } break;           \
+    case LBM_STORE_F16: { using S = __half; BODY; } break;          \
+    default: { using S = __hip_bfloat16; BODY; } break;             \
+  }
+
+int lbm_particles_upload_belt2() {
+  int8_t t[98][3];
+  int l = 0;
+  for (int k = -2; k <= 2; ++k)
+    for (int j = -2; j <= 2; ++j)
+      for (int i = -2; i <= 2; ++i)
+        if (i == -2 || i == 2 || j == -2 || j == 2 || k == -2 || k == 2) {
+          t[l][0] = (int8_t)i; t[l][1] = (int8_t)j; t[l][2] = (int8_t)k;
+          ++l;
+        }
+  return hipMemcpyToSymbol(HIP_SYMBOL(C_BELT2), t, sizeof t) == hipSuccess ? 0 : LBM_ERR_HIP;
+}
+
+int lbm_particles_flags(lbm_ctx* c, int level) {
+  PartArgs a = make_pargs(c);
+  a.pos_cur = c->d_pos[level];
+  a.fl_cur = c->flags[level];
+  a.key_cur = c->d_key[level];
+  a.stamp_cur = c->d_stamp[level];
+  hipLaunchKernelGGL(k_unstamp, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  hipLaunchKernelGGL(k_stamp, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  hipLaunchKernelGGL(k_resolve, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  return last_err();
+}
+
+int lbm_particles_erase(lbm_ctx* c) {
+  PartArgs a = make_pargs(c);
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_erase<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  return last_err();
+}
+
+int lbm_particles_fill(lbm_ctx* c) {
+  PartArgs a = make_pargs(c);
+  hipLaunchKernelGGL(k_fill_claim, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_fill_apply<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  hipLaunchKernelGGL(k_fill_release, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  return last_err();
+}
+
+int lbm_particles_wetting(lbm_ctx* c) {
+  if (c->cfg.wetting == LBM_WET_NONE) return 0;
+  PartArgs a = make_pargs(c);
+  hipLaunchKernelGGL(k_wetting, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  return last_err();
+}
+
+int lbm_particles_bounce(lbm_ctx* c) {
+  PartArgs a = make_pargs(c);
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_bounce<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  return last_err();
+}
+
+int lbm_particles_integrate(lbm_ctx* c) {
+  PartArgs a = make_pargs(c);
+  const lbm_config& g = c->cfg;
+  PairParams pp;
+  pp.hz_k = g.hz_k; pp.hz_rmin = g.hz_rmin; pp.hz_rcap = g.hz_rcap;
+  pp.gmin = 2.5 * g.hz_k * pow((double)g.hz_rmin - (double)g.hz_rcap, 1.5);
+  pp.rpar = 2.0 * (double)g.radius;
+  const double R = g.radius;
+  pp.lubfactor = (double)g.lub_k * 6.0 * 3.14159265358979323846 * (R * R) * (R * R) / (pp.rpar * pp.rpar);
+  pp.rparcut = pp.rpar + (double)g.lub_rmin;
+  pp.rparcap = pp.rpar + (double)g.lub_rcap;
+  pp.mxrsqcut = fmax(pp.rparcut * pp.rparcut, (double)g.hz_rmin * (double)g.hz_rmin);
+  pp.visc = ((double)g.tau_r - 0.5) / 3.0;
+  pp.hertz = g.hertz;
+  pp.lub = g.lubrication;
+  unsigned nb = (unsigned)((c->np + TPB - 1) / TPB);
+  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(TPB), 0, c->stream, a, pp);
+  IntParams ip;
+  ip.rmass = g.rmass;
+  ip.rotin = g.rmass > 0 ? 2.0 / 5.0 * (1.0 / (double)g.rmass) * R * R : 0.0;
+  ip.max_vel = g.max_vel;
+  for (int d = 0; d < 3; ++d) { ip.ext_f[d] = g.ext_force[d]; ip.ext_t[d] = g.lrotate ? g.ext_torque[d] : 0.0; }
+  hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(TPB), 0, c->stream, a, ip);
+  return last_err();
+}

@@ -1,0 +1,196 @@
+"""ctypes binding of liblbcuda_amd.so, the C ABI of include/lbcuda_amd.h.
+
+Box mirrors the reference host program's view of one simulation box: create
+(setupGPUgridAndAlloc + SetConstants, main.CUF:2133-2407, 1043-1162), initial
+fields (setupRho_Pops, main.CUF:2531-2608), particles (read_input_atom +
+SetupParts, main.CUF:1187-1300) and the time step (main.CUF:127-278).  Arrays
+are numpy arrays in Fortran order: pops (nx,ny,nz,19), rho (nx,ny,nz),
+vel (3,nx,ny,nz), pos (13,n), force (12,n).
+
+There is no CPU fallback: if the shared library is missing or cannot be
+loaded, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .abi import LbmConfig, ERR_NAMES
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "liblbcuda_amd.so")
+_LIB = None
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code, what, msg=""):
+        self.code = code
+        super().__init__(f"{what}: {ERR_NAMES.get(code, code)} {msg}".strip())
+
+
+def lib():
+    """Load liblbcuda_amd.so (built by lbcuda_amd.build / __graft_entry__.build)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    i32 = ctypes.c_int32
+    L.lbm_config_default.argtypes = [P(LbmConfig), i32]
+    L.lbm_config_default.restype = None
+    L.lbm_create.argtypes = [P(LbmConfig), P(vp)]
+    L.lbm_destroy.argtypes = [vp]
+    L.lbm_init_fields.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.lbm_set_pops.argtypes = [vp, vp, vp]
+    L.lbm_set_particles.argtypes = [vp, i32, vp]
+    L.lbm_step.argtypes = [vp, i32]
+    L.lbm_get_pops.argtypes = [vp, i32, vp]
+    L.lbm_get_rho.argtypes = [vp, vp, vp]
+    L.lbm_get_moments.argtypes = [vp, vp, vp, vp, vp]
+    L.lbm_get_flags.argtypes = [vp, vp]
+    L.lbm_get_particles.argtypes = [vp, vp, vp]
+    L.lbm_get_sums.argtypes = [vp, vp]
+    L.lbm_status.argtypes = [vp, P(i32), P(i32), ctypes.c_char_p, i32]
+    L.lbm_kernel_timing.argtypes = [vp, i32]
+    L.lbm_get_kernel_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_int64)]
+    L.lbm_kernel_info.argtypes = [vp, ctypes.c_char_p, i32, P(ctypes.c_double)]
+    L.lbm_version.restype = ctypes.c_char_p
+    for name in ("lbm_create", "lbm_destroy", "lbm_init_fields", "lbm_set_pops", "lbm_set_particles",
+                 "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments", "lbm_get_flags",
+                 "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
+                 "lbm_get_kernel_timing", "lbm_kernel_info"):
+        getattr(L, name).restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+EXPORTED = ["lbm_config_default", "lbm_create", "lbm_destroy", "lbm_init_fields", "lbm_set_pops",
+            "lbm_set_particles", "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments",
+            "lbm_get_flags", "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
+            "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version"]
+
+
+def _f32(a):
+    return None if a is None else np.asfortranarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Box:
+    """One periodic box on the GPU."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.n = (cfg.nx, cfg.ny, cfg.nz)
+        self.h = ctypes.c_void_p()
+        self._check(lib().lbm_create(ctypes.byref(cfg), ctypes.byref(self.h)), "lbm_create")
+        self.npart = 0
+
+    def close(self):
+        if self.h:
+            lib().lbm_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise LbmError(rc, what)
+
+    def init_fields(self, init_type, u0=(0.0, 0.0, 0.0), rho_r=None, rho_b=None, vel=None):
+        u = np.asarray(u0, dtype=np.float32)
+        rr, rb, v = _f32(rho_r), _f32(rho_b), _f32(vel)
+        self._check(lib().lbm_init_fields(self.h, init_type, _ptr(u), _ptr(rr), _ptr(rb), _ptr(v)),
+                    "lbm_init_fields")
+
+    def set_pops(self, pops_r, pops_b):
+        a, b = _f32(pops_r), _f32(pops_b)
+        self._check(lib().lbm_set_pops(self.h, _ptr(a), _ptr(b)), "lbm_set_pops")
+
+    def set_particles(self, pos13):
+        p = np.asfortranarray(pos13, dtype=np.float64)
+        self.npart = p.shape[1]
+        self._check(lib().lbm_set_particles(self.h, self.npart, _ptr(p)), "lbm_set_particles")
+
+    def step(self, n=1):
+        """Advance n steps; returns the device error code (0 if none)."""
+        rc = lib().lbm_step(self.h, n)
+        if rc in (1, 2, 3, 9):
+            raise LbmError(rc, "lbm_step")
+        return rc
+
+    def pops(self, color):
+        out = np.zeros(self.n + (19,), dtype=np.float32, order="F")
+        self._check(lib().lbm_get_pops(self.h, color, _ptr(out)), "lbm_get_pops")
+        return out
+
+    def rho(self):
+        a = np.zeros(self.n, dtype=np.float32, order="F")
+        b = np.zeros(self.n, dtype=np.float32, order="F")
+        self._check(lib().lbm_get_rho(self.h, _ptr(a), _ptr(b)), "lbm_get_rho")
+        return a, b
+
+    def moments(self):
+        rr = np.zeros(self.n, dtype=np.float32, order="F")
+        rb = np.zeros(self.n, dtype=np.float32, order="F")
+        vel = np.zeros((3,) + self.n, dtype=np.float32, order="F")
+        ph = np.zeros(self.n, dtype=np.float32, order="F")
+        self._check(lib().lbm_get_moments(self.h, _ptr(rr), _ptr(rb), _ptr(vel), _ptr(ph)), "lbm_get_moments")
+        return rr, rb, vel, ph
+
+    def flags(self):
+        out = np.zeros(self.n, dtype=np.int8, order="F")
+        self._check(lib().lbm_get_flags(self.h, _ptr(out)), "lbm_get_flags")
+        return out
+
+    def particles(self):
+        pos = np.zeros((13, self.npart), dtype=np.float64, order="F")
+        frc = np.zeros((12, self.npart), dtype=np.float64, order="F")
+        self._check(lib().lbm_get_particles(self.h, _ptr(pos), _ptr(frc)), "lbm_get_particles")
+        return pos, frc
+
+    def sums(self):
+        out = np.zeros(5, dtype=np.float64)
+        self._check(lib().lbm_get_sums(self.h, _ptr(out)), "lbm_get_sums")
+        return out
+
+    def status(self):
+        code = ctypes.c_int32()
+        step = ctypes.c_int32()
+        buf = ctypes.create_string_buffer(256)
+        self._check(lib().lbm_status(self.h, ctypes.byref(code), ctypes.byref(step), buf, 256), "lbm_status")
+        return code.value, step.value, buf.value.decode()
+
+    def kernel_timing(self, on=True):
+        self._check(lib().lbm_kernel_timing(self.h, 1 if on else 0), "lbm_kernel_timing")
+
+    def get_kernel_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(lib().lbm_get_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)), "lbm_get_kernel_timing")
+        return ms.value, n.value
+
+    def kernel_info(self):
+        buf = ctypes.create_string_buffer(128)
+        b = ctypes.c_double()
+        self._check(lib().lbm_kernel_info(self.h, buf, 128, ctypes.byref(b)), "lbm_kernel_info")
+        return buf.value.decode(), b.value
+
+
+def version():
+    return lib().lbm_version().decode()

@@ -1,0 +1,186 @@
+"""GPU parity: the HIP library, called through its C ABI, against the FP64
+CPU oracle on the same seeded inputs.
+
+Tolerances (FP32 arithmetic on the GPU vs the FP64 oracle):
+  populations / densities  |d| <= 2e-5 after tens of steps (values <= 1);
+  node flags               bit-exact (integer work);
+  particle positions       |d| <= 1e-6 lattice units;
+  particle forces          |d| <= 1e-4 * (1 + |F|).
+Shifted 16-bit storage: relative density error vs the FP64 oracle
+<= 1e-3 (fp16) and <= 1e-2 (bf16).
+Populations on non-fluid nodes are not compared: there the reference keeps
+stale values that never reach the fluid (no collision, :689).
+"""
+import numpy as np
+import pytest
+
+from lbcuda_amd.abi import (default_config, INIT_RHO, INIT_VELFIELD, INIT_DROPLET, INIT_ZPLANES,
+                            INIT_CYLINDER, WET_NEUTRAL, WET_JANUS, STORE_F16, STORE_BF16)
+from lbcuda_amd import workloads as wl
+from lbcuda_amd.lib import Box
+from oracle.pyoracle import OracleBox
+
+pytestmark = pytest.mark.gpu
+
+
+def both(cfg, init, pos=None, **kw):
+    g = Box(cfg)
+    o = OracleBox(cfg, 64)
+    if pos is not None:
+        g.set_particles(pos)
+        o.set_particles(pos)
+    g.init_fields(init, **kw)
+    o.init_fields(init, **kw)
+    return g, o
+
+
+def maxdiff(a, b, mask=None):
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    if mask is not None:
+        d = d[mask]
+    return float(d.max()) if d.size else 0.0
+
+
+@pytest.mark.parametrize("shape", [(24, 24, 24), (40, 20, 12), (64, 64, 16)])
+def test_spinodal_parity(shape):
+    print(f"spinodal parity shape={shape}", flush=True)
+    cfg = default_config(shape[0], sigma=0.02, beta=0.7)
+    cfg.nx, cfg.ny, cfg.nz = shape
+    rr, rb = wl.spinodal_rho(shape, seed=101)
+    g, o = both(cfg, INIT_RHO, rho_r=rr, rho_b=rb)
+    assert g.step(30) == 0 and o.step(30) == 0
+    for c in (0, 1):
+        assert maxdiff(g.pops(c), o.pops(c)) < 2e-5
+    gr, gb = g.rho()
+    orr, ob = o.rho()
+    assert maxdiff(gr, orr) < 2e-5 and maxdiff(gb, ob) < 2e-5
+
+
+@pytest.mark.parametrize("init", [INIT_DROPLET, INIT_ZPLANES, INIT_CYLINDER])
+def test_init_types_parity(init):
+    print(f"init type {init}", flush=True)
+    n = 32
+    cfg = default_config(n, sigma=0.05, beta=0.7)
+    g, o = both(cfg, init, u0=(0.01, 0.0, -0.01))
+    for c in (0, 1):
+        assert maxdiff(g.pops(c), o.pops(c)) < 1e-6
+    assert g.step(20) == 0 and o.step(20) == 0
+    for c in (0, 1):
+        assert maxdiff(g.pops(c), o.pops(c)) < 2e-5
+
+
+def test_taylor_green_parity():
+    print("taylor-green parity", flush=True)
+    n = 32
+    rr, rb, vel = wl.taylor_green(n, u0=0.01)
+    cfg = default_config(n)
+    g, o = both(cfg, INIT_VELFIELD, rho_r=rr, rho_b=rb, vel=vel)
+    assert g.step(100) == 0 and o.step(100) == 0
+    assert maxdiff(g.pops(0), o.pops(0)) < 1e-6
+    rrg, _, velg, _ = g.moments()
+    f = o.pops(0)
+    ex = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0])
+    u_o = (f * ex).sum(-1) / f.sum(-1)
+    assert maxdiff(velg[0], u_o) < 1e-7
+
+
+def test_global_sums_fp64():
+    print("global sums", flush=True)
+    n = 32
+    rr, rb = wl.spinodal_rho((n, n, n), seed=5)
+    cfg = default_config(n, sigma=0.02, beta=0.7)
+    g, o = both(cfg, INIT_RHO, u0=(0.01, 0.0, 0.0), rho_r=rr, rho_b=rb)
+    s0 = g.sums()
+    g.step(50)
+    s1 = g.sums()
+    assert abs(s1[0] - s0[0]) < 1e-3 * 1e-3 * s0[0]   # mass per colour, FP32 steps
+    assert abs(s1[1] - s0[1]) < 1e-3 * 1e-3 * s0[1]
+    assert np.allclose(s1[2:], s0[2:], atol=1e-3)
+    o.step(50)
+    assert abs(s1[0] - o.pops(0).sum()) < 1e-6 * s0[0]
+
+
+def test_density_range_guard():
+    """init_rhoR_CG range guard (kernels_fluid_CG.CUF:438-441) -> error code 4."""
+    n = 16
+    rr = np.full((n, n, n), 20.0, np.float32)
+    rb = np.zeros_like(rr)
+    g = Box(default_config(n))
+    g.init_fields(INIT_RHO, rho_r=rr, rho_b=rb)
+    assert g.step(1) == 4
+    code, step, msg = g.status()
+    assert code == 4 and step == 1 and "438" in msg
+
+
+def particle_case(n=32, npart=3, seed=11, lrotate=1, rmass=1.0 / 1500.0, vel=0.05, wetting=WET_NEUTRAL):
+    cfg = default_config(n, sigma=0.02, beta=0.7, with_particles=1, n_particles=npart, lrotate=lrotate,
+                         rmass=rmass, max_vel=1.0, wetting=wetting)
+    pos = wl.random_spheres((n, n, n), npart, 5.5, seed=seed)
+    rng = np.random.default_rng(seed)
+    pos[3:6] = vel * rng.uniform(-1, 1, (3, npart))
+    if lrotate:
+        pos[6:9] = 0.01 * rng.uniform(-1, 1, (3, npart))
+    rr, rb = wl.spinodal_rho((n, n, n), seed=seed)
+    return cfg, pos, rr, rb
+
+
+@pytest.mark.parametrize("wetting,lrotate", [(WET_NEUTRAL, 1), (WET_JANUS, 1), (WET_NEUTRAL, 0)])
+def test_particles_parity(wetting, lrotate):
+    print(f"particles parity wetting={wetting} lrotate={lrotate}", flush=True)
+    cfg, pos, rr, rb = particle_case(npart=3, lrotate=lrotate, wetting=wetting)
+    g, o = both(cfg, INIT_RHO, pos=pos, rho_r=rr, rho_b=rb)
+    assert (g.flags() == o.flags()).all()
+    for step in range(4):
+        assert g.step(10) == 0 and o.step(10) == 0
+        fl = o.flags()
+        assert (g.flags() == fl).all(), f"flags differ after {10 * (step + 1)} steps"
+        fluid = fl == 1
+        for c in (0, 1):
+            assert maxdiff(g.pops(c), o.pops(c), fluid[..., None].repeat(19, -1)) < 2e-5
+        gr, gb = g.rho()
+        orr, ob = o.rho()
+        assert maxdiff(gr, orr, fluid) < 2e-5 and maxdiff(gb, ob, fluid) < 2e-5
+        gp, gf = g.particles()
+        op, of = o.particles()
+        assert maxdiff(gp[0:6], op[0:6]) < 1e-6
+        assert maxdiff(gp[6:13], op[6:13]) < 1e-6
+        assert np.all(np.abs(gf - of) <= 1e-4 * (1 + np.abs(of)))
+    p_end, _ = o.particles()
+    assert np.abs(p_end[0:3] - pos[0:3]).max() > 0.5   # the particles crossed nodes
+
+
+def test_prescribed_velocity_sphere_parity():
+    """A sphere with 1/m = 0 keeps its velocity (config: translating sphere):
+    fill/erase every few steps, drag history vs the oracle."""
+    print("prescribed sphere", flush=True)
+    n = 32
+    cfg = default_config(n, sigma=0.0, beta=0.0, with_particles=1, n_particles=1, rmass=0.0, max_vel=1.0,
+                         wetting=WET_NEUTRAL)
+    pos = np.zeros((13, 1), order="F")
+    pos[0:3, 0] = (15.7, 16.2, 16.0)
+    pos[3, 0] = 0.05
+    pos[9, 0] = 1.0
+    rr = np.ones((n, n, n), np.float32)
+    rb = np.zeros_like(rr)
+    g, o = both(cfg, INIT_RHO, pos=pos, rho_r=rr, rho_b=rb)
+    for _ in range(6):
+        assert g.step(10) == 0 and o.step(10) == 0
+        assert (g.flags() == o.flags()).all()
+        _, gf = g.particles()
+        _, of = o.particles()
+        assert np.all(np.abs(gf - of) <= 1e-4 * (1 + np.abs(of)))
+
+
+@pytest.mark.parametrize("storage,tol", [(STORE_F16, 1e-3), (STORE_BF16, 1e-2)])
+def test_low_precision_storage_error(storage, tol):
+    print(f"storage {storage}", flush=True)
+    n = 32
+    cfg = default_config(n, sigma=0.02, beta=0.7, storage=storage, store_rho0=0.5)
+    rr, rb = wl.spinodal_rho((n, n, n), seed=77)
+    g, o = both(cfg, INIT_RHO, rho_r=rr, rho_b=rb)
+    assert g.step(100) == 0 and o.step(100) == 0
+    orr, ob = o.rho()
+    gr, gb = g.rho()
+    rel = max(maxdiff(gr, orr), maxdiff(gb, ob)) / float(np.abs(orr + ob).max())
+    print(f"storage {storage}: max rel density error {rel:.3e}")
+    assert rel < tol
