@@ -167,6 +167,15 @@ int lbm_get_kernel_timing(lbm_ctx* ctx, double* total_ms, int64_t* launches);
 int lbm_kernel_info(lbm_ctx* ctx, char* name, int32_t name_len,
                     double* bytes_per_lu);
 
+/* Host-only helper (no GPU work): the seeded random colour field of
+ * initType 6 (set_random_dens_fluids, main.CUF:2610-2684) in reproducible
+ * form: rho_r = mean + amp*(2u-1), u = SplitMix64(seed ^ node)/2^64 with
+ * node = x + nx*(y + ny*z), rho_b = 1 - rho_r.  The reference draws u from
+ * the compiler's RANDOM_NUMBER with its default seed, which no other host can
+ * reproduce.  rho_r/rho_b are (nx,ny,nz) host arrays. */
+int lbm_fill_random_rho(uint64_t seed, double mean, double amp, int32_t nx, int32_t ny,
+                        int32_t nz, float* rho_r, float* rho_b);
+
 /* Library build identification (gfx target, storage formats). */
 const char* lbm_version(void);
 

@@ -131,6 +131,22 @@ void lbm_config_default(lbm_config* cfg, int32_t n) {
   cfg->lub_k = 10.0f; cfg->lub_rmin = 10.0f; cfg->lub_rcap = 9.0f;
 }
 
+int lbm_fill_random_rho(uint64_t seed, double mean, double amp, int32_t nx, int32_t ny, int32_t nz,
+                        float* rho_r, float* rho_b) {
+  if (!rho_r || !rho_b || nx <= 0 || ny <= 0 || nz <= 0) return LBM_ERR_ARG;
+  const uint64_t n = (uint64_t)nx * (uint64_t)ny * (uint64_t)nz;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t z = (seed ^ i) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    rho_r[i] = (float)(mean + amp * u);
+    rho_b[i] = 1.0f - rho_r[i];
+  }
+  return LBM_OK;
+}
+
 const char* lbm_version(void) {
   return "lbcuda_amd 0.1 gfx950 (storage: f32, f16-shifted, bf16-shifted)";
 }

@@ -58,6 +58,8 @@ def lib():
     L.lbm_get_kernel_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_int64)]
     L.lbm_kernel_info.argtypes = [vp, ctypes.c_char_p, i32, P(ctypes.c_double)]
     L.lbm_version.restype = ctypes.c_char_p
+    L.lbm_fill_random_rho.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double, i32, i32, i32, vp, vp]
+    L.lbm_fill_random_rho.restype = ctypes.c_int
     for name in ("lbm_create", "lbm_destroy", "lbm_init_fields", "lbm_set_pops", "lbm_set_particles",
                  "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments", "lbm_get_flags",
                  "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
@@ -70,7 +72,7 @@ def lib():
 EXPORTED = ["lbm_config_default", "lbm_create", "lbm_destroy", "lbm_init_fields", "lbm_set_pops",
             "lbm_set_particles", "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments",
             "lbm_get_flags", "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
-            "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version"]
+            "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version", "lbm_fill_random_rho"]
 
 
 def _f32(a):
@@ -190,6 +192,17 @@ class Box:
         b = ctypes.c_double()
         self._check(lib().lbm_kernel_info(self.h, buf, 128, ctypes.byref(b)), "lbm_kernel_info")
         return buf.value.decode(), b.value
+
+
+def fill_random_rho(shape, seed, mean=0.5, amp=0.05):
+    """Host-side seeded colour field of the library (no GPU work)."""
+    nx, ny, nz = shape
+    rr = np.zeros(shape, np.float32, order="F")
+    rb = np.zeros(shape, np.float32, order="F")
+    rc = lib().lbm_fill_random_rho(seed, mean, amp, nx, ny, nz, _ptr(rr), _ptr(rb))
+    if rc:
+        raise LbmError(rc, "lbm_fill_random_rho")
+    return rr, rb
 
 
 def version():
