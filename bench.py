@@ -54,6 +54,9 @@ def run_gpu(args, storage, rr, rb, pos):
     if args.particles:
         box.set_particles(pos)
     box.init_fields(INIT_RHO, rho_r=rr, rho_b=rb)
+    if not args.no_autotune:
+        t, zc, ms = box.autotune(reps=3)
+        log(f"storage={storage}: autotune picked tile {t} z-chunk {zc} ({ms:.3f} ms per launch)")
     log(f"storage={storage}: warmup {args.warmup} steps")
     rc = box.step(args.warmup)
     if rc:
@@ -113,6 +116,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one box on one GPU; multi-GPU runs are not part of it "
@@ -123,6 +127,9 @@ def main():
     log(f"inputs: {args.n}^3, {args.particles} spheres")
     rr, rb, pos = wl.pickering_inputs(args.n, args.particles)
 
+    from lbcuda_amd.lib import copy_bandwidth
+    copy_gbs = copy_bandwidth(2 << 30, 10)
+    log(f"device copy kernel: {copy_gbs:.0f} GB/s (read + write)")
     r32 = run_gpu(args, 0, rr, rb, pos)
     r16 = None if args.no_fp16 else run_gpu(args, 1, rr, rb, pos)
     cpu = None if args.no_cpu else cpu_baseline(args, rr, rb, pos)
@@ -150,6 +157,7 @@ def main():
                      "traffic": traffic, "kernel": r32["kernel"], "bytes_per_lu": r32["bytes_per_lu"],
                      "kernel_ms": round(r32["kernel_ms"], 4)},
         "mlups_ref_style": round(r32["mlups"] * ((args.n + 2) / args.n) ** 3, 1),
+        "copy_kernel_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
     }
     if r16:

@@ -48,6 +48,12 @@ extern "C" {
 #define LBM_INIT_VELFIELD  7   /* rho and per-node velocity from host (no reference
                                   counterpart: needed for Taylor-Green starts)     */
 
+/* Collide-stream kernel variants (lbm_set_kernel). */
+#define LBM_KERNEL_FUSED   0   /* one pass: densities, phase field, collision and
+                                  push streaming, z-marching tiles (default)    */
+#define LBM_KERNEL_TWOPASS 1   /* density pre-pass + collide-push, the
+                                  reference's init_rho*_CG / time_step_CG split */
+
 /* Error codes (the reference stores __LINE__ in stop_d and aborts). */
 #define LBM_OK               0
 #define LBM_ERR_ARG          1   /* bad argument / config                      */
@@ -60,6 +66,9 @@ extern "C" {
 #define LBM_ERR_PART_SETUP   8   /* main.CUF:1207-1221 (SetupParts checks)      */
 #define LBM_ERR_STATE        9   /* call order (e.g. step before init)          */
 #define LBM_ERR_BATCH       10   /* batched launch configuration                */
+#define LBM_ERR_LIST_OVERFLOW 11 /* more than 32 particles near one particle
+                                    (the reference stops on >1000 particles per
+                                    tile list, kernels_fluid_PART.CUF:2436-2439) */
 
 /*
  * Run-time configuration.  The reference fixes most of these at compile time
@@ -72,7 +81,7 @@ typedef struct lbm_config {
   float   sigma;             /* surface tension sigma_CG (dimensions_m:50)        */
   float   beta;              /* recolouring beta_CG (dimensions_m:50)             */
   int32_t storage;           /* LBM_STORE_*                                       */
-  float   store_rho0;        /* rho0 of the shifted low-precision storage         */
+  float   store_rho0;        /* rho0 of the shifted low-precision storage, 0..1.2 */
   float   min_rho, max_rho;  /* density range guard (kernels_fluid.CUF:35)        */
   /* particles (dimensions_m.CUF:55-84) */
   int32_t with_particles;
@@ -162,6 +171,27 @@ int lbm_status(lbm_ctx* ctx, int32_t* code, int32_t* step, char* msg,
 int lbm_kernel_timing(lbm_ctx* ctx, int32_t on);
 int lbm_get_kernel_timing(lbm_ctx* ctx, double* total_ms, int64_t* launches);
 
+/* Kernel choice.  The reference fixes the thread-block tile at compile time
+ * (TILE1..TILE3, defines.h:14-24, checked in main.CUF:2137-2148); here the
+ * variant (LBM_KERNEL_*), the fused kernel's tile (index into the built-in
+ * tile table, see lbm_tile_shape) and its z-chunk length are run-time
+ * choices.  lbm_set_kernel rejects a tile that does not fit the box. */
+int lbm_set_kernel(lbm_ctx* ctx, int32_t variant, int32_t tile, int32_t zchunk);
+int lbm_get_kernel(lbm_ctx* ctx, int32_t* variant, int32_t* tile, int32_t* zchunk);
+
+/* Tile table of the fused kernel: number of entries, and the shape of entry
+ * tile (x width, y height, wrap = 1 if the tile spans the whole x extent and
+ * is only valid when nx equals the width). */
+int32_t lbm_num_tiles(void);
+int lbm_tile_shape(int32_t tile, int32_t* tx, int32_t* ty, int32_t* wrap);
+
+/* Autotuner (no reference counterpart): times every valid (tile, z chunk)
+ * of the fused kernel on this box's current state, reps launches each, into
+ * a scratch level (the state is not advanced or changed), keeps the fastest
+ * and returns it with its mean launch time in milliseconds.  Call after
+ * lbm_init_fields (and lbm_set_particles). */
+int lbm_autotune(lbm_ctx* ctx, int32_t reps, int32_t* tile, int32_t* zchunk, double* best_ms);
+
 /* Name of the collide-stream kernel variant in use and its algorithmic bytes
  * per lattice update (2 colours x 19 pops x read+write x bytes/value + flag). */
 int lbm_kernel_info(lbm_ctx* ctx, char* name, int32_t name_len,
@@ -175,6 +205,12 @@ int lbm_kernel_info(lbm_ctx* ctx, char* name, int32_t name_len,
  * reproduce.  rho_r/rho_b are (nx,ny,nz) host arrays. */
 int lbm_fill_random_rho(uint64_t seed, double mean, double amp, int32_t nx, int32_t ny,
                         int32_t nz, float* rho_r, float* rho_b);
+
+/* Device-to-device copy bandwidth in GB/s (read + write bytes over time) of
+ * a plain copy kernel over two buffers of the given size, reps launches.
+ * The reference probes host transfer bandwidth at start-up
+ * (main.CUF:2396-2404); this is the HBM ceiling the step is compared with. */
+int lbm_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs);
 
 /* Library build identification (gfx target, storage formats). */
 const char* lbm_version(void);

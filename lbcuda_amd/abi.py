@@ -7,13 +7,14 @@ import ctypes
 
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 WET_NONE, WET_JANUS, WET_NEUTRAL = 0, 1, 2
+KERNEL_FUSED, KERNEL_TWOPASS = 0, 1
 INIT_DROPLET, INIT_ZPLANES, INIT_RHO, INIT_POPS, INIT_CYLINDER, INIT_RANDOM, INIT_VELFIELD = range(1, 8)
 
 ERR_NAMES = {
     0: "LBM_OK", 1: "LBM_ERR_ARG", 2: "LBM_ERR_ALLOC", 3: "LBM_ERR_HIP",
     4: "LBM_ERR_RHO_RANGE", 5: "LBM_ERR_BIG_FORCE", 6: "LBM_ERR_EXTREME_VEL",
     7: "LBM_ERR_OUT_OF_BOX", 8: "LBM_ERR_PART_SETUP", 9: "LBM_ERR_STATE",
-    10: "LBM_ERR_BATCH",
+    10: "LBM_ERR_BATCH", 11: "LBM_ERR_LIST_OVERFLOW",
 }
 
 

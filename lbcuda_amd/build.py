@@ -6,12 +6,13 @@ Packed FP32 instructions are disabled for all device code
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "liblbcuda_amd.so")
-SOURCES = ["lbm_fluid.hip", "lbm_particles.hip", "lbm_api.cpp"]
+SOURCES = ["lbm_fused.hip", "lbm_fluid.hip", "lbm_particles.hip", "lbm_api.cpp"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 
@@ -23,17 +24,29 @@ def hipcc():
 
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
-    objs = []
     common = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
               "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
+    cmds = []
+    objs = []
     for src in SOURCES:
         obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-        cmd = common + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmds.append(common + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
-        objs.append(obj)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        out = "\n".join(x for x in r.stdout.splitlines() if "not a recognized feature" not in x)
+        if out.strip():
+            print(out, flush=True)
+        return r.returncode
+
+    with ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as ex:
+        codes = list(ex.map(run, cmds))
+    if any(codes):
+        raise RuntimeError("hipcc failed")
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

@@ -15,6 +15,8 @@
  */
 #include "lbm_common.h"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -95,6 +97,49 @@ void build_ball(int rmax, std::vector<int8_t>& ball) {
       }
 }
 
+/* D3Q27 ring of the template's surface nodes (offsets outside the template):
+ * the fluid nodes whose densities the particle kernels read; z, y, x order
+ * so consecutive threads read consecutive x */
+void build_ring(const std::vector<int8_t>& tmpl, std::vector<int8_t>& ring) {
+  std::vector<int> seen;
+  const size_t nt = tmpl.size() / 4;
+  auto key = [](int i, int j, int k) { return (i + 32) + 64 * ((j + 32) + 64 * (k + 32)); };
+  for (size_t t = 0; t < nt; ++t) seen.push_back(key(tmpl[4 * t], tmpl[4 * t + 1], tmpl[4 * t + 2]));
+  std::vector<std::array<int, 3>> out;
+  for (size_t t = 0; t < nt; ++t) {
+    if (tmpl[4 * t + 3] != lbm::FL_LIST) continue;
+    for (int k = -1; k <= 1; ++k)
+      for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) {
+          if (!(i || j || k)) continue;
+          const int a = tmpl[4 * t] + i, b = tmpl[4 * t + 1] + j, cc = tmpl[4 * t + 2] + k;
+          const int q = key(a, b, cc);
+          if (std::find(seen.begin(), seen.end(), q) != seen.end()) continue;
+          seen.push_back(q);
+          out.push_back({a, b, cc});
+        }
+  }
+  std::sort(out.begin(), out.end(), [](const std::array<int, 3>& u, const std::array<int, 3>& v) {
+    if (u[2] != v[2]) return u[2] < v[2];
+    if (u[1] != v[1]) return u[1] < v[1];
+    return u[0] < v[0];
+  });
+  ring.clear();
+  for (auto& e : out) {
+    ring.push_back((int8_t)e[0]); ring.push_back((int8_t)e[1]); ring.push_back((int8_t)e[2]); ring.push_back(0);
+  }
+}
+
+/* default tile for the fused kernel: the whole-row tile when one fits */
+void default_kernel(lbm_ctx* c) {
+  c->variant = LBM_KERNEL_FUSED;
+  c->tile = 0;
+  const int prefer[] = {5, 7, 9};   /* 256x2, 128x4, 64x4 with x wrap */
+  for (int t : prefer)
+    if (lbm_fused_tile_valid(c, t, 1)) { c->tile = t; break; }
+  c->zchunk = c->nz < 32 ? c->nz : 32;
+}
+
 int restamp_initial(lbm_ctx* c) {
   /* step-0 node flags (setupRho_Pops, main.CUF:2596-2605): the current level
      from the positions, the other level stays all fluid. */
@@ -157,7 +202,17 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
   if (cfg->nx < 3 || cfg->ny < 3 || cfg->nz < 3) return LBM_ERR_ARG;
   if (cfg->tau_r <= 0.5f || cfg->tau_b <= 0.5f) return LBM_ERR_ARG;
   if (cfg->storage < LBM_STORE_F32 || cfg->storage > LBM_STORE_BF16) return LBM_ERR_ARG;
+  /* shifted 16-bit storage: the fused kernel re-stores non-fluid values
+     through float, exact while w_0 rho0 + |f - w rho0| < 0.5 */
+  if (cfg->storage != LBM_STORE_F32 && !(cfg->store_rho0 >= 0.0f && cfg->store_rho0 <= 1.2f)) return LBM_ERR_ARG;
   if (cfg->with_particles && (cfg->radius < 1.0f || cfg->radius > 30.0f)) return LBM_ERR_ARG;
+  /* particle kernels wrap offsets of up to rmax + 2 nodes once per axis */
+  if (cfg->with_particles) {
+    const int rmx = (int)(cfg->radius + 1.0f) + 3;
+    if (cfg->nx < rmx || cfg->ny < rmx || cfg->nz < rmx) return LBM_ERR_ARG;
+  }
+  /* 32-bit node offsets inside the kernels */
+  if ((double)cfg->nx * cfg->ny * cfg->nz > (double)(1u << 30)) return LBM_ERR_ARG;
   lbm_ctx* c = new (std::nothrow) lbm_ctx;
   if (!c) return LBM_ERR_ALLOC;
   std::memset((void*)c, 0, sizeof *c);
@@ -166,6 +221,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
   c->N = (size_t)c->nx * c->ny * c->nz;
   c->storage = cfg->storage;
   c->esize = cfg->storage == LBM_STORE_F32 ? 4 : 2;
+  default_kernel(c);
   int rc = 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return LBM_ERR_HIP; }
   for (int b = 0; b < 2; ++b) {
@@ -178,18 +234,27 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
   rc |= dalloc(&c->d_status, 1);
   if (!rc) rc |= hip_ok(hipMemset(c->d_status, 0, sizeof(lbm::DevStatus)));
   if (!rc && cfg->with_particles) {
-    std::vector<int8_t> tmpl, ball;
+    std::vector<int8_t> tmpl, ball, ring, tlist;
     build_template(cfg->radius, &c->rmax, tmpl);
     build_ball(c->rmax, ball);
+    build_ring(tmpl, ring);
+    for (size_t t = 0; t < tmpl.size() / 4; ++t)
+      if (tmpl[4 * t + 3] == lbm::FL_LIST) tlist.insert(tlist.end(), &tmpl[4 * t], &tmpl[4 * t] + 4);
     c->ntmpl = (int)(tmpl.size() / 4);
     c->nball = (int)(ball.size() / 4);
+    c->nring = (int)(ring.size() / 4);
+    c->ntlist = (int)(tlist.size() / 4);
     rc |= dalloc(&c->d_tmpl, tmpl.size());
+    rc |= dalloc(&c->d_ring, ring.size());
+    rc |= dalloc(&c->d_tlist, tlist.size());
     rc |= dalloc(&c->d_ball, ball.size());
     for (int b = 0; b < 2; ++b) rc |= dalloc(&c->d_key[b], c->N);
     rc |= dalloc(&c->d_claim, c->N);
     if (!rc) {
       rc |= hip_ok(hipMemcpy(c->d_tmpl, tmpl.data(), tmpl.size(), hipMemcpyHostToDevice));
       rc |= hip_ok(hipMemcpy(c->d_ball, ball.data(), ball.size(), hipMemcpyHostToDevice));
+      rc |= hip_ok(hipMemcpy(c->d_ring, ring.data(), ring.size(), hipMemcpyHostToDevice));
+      rc |= hip_ok(hipMemcpy(c->d_tlist, tlist.data(), tlist.size(), hipMemcpyHostToDevice));
       for (int b = 0; b < 2; ++b) rc |= hip_ok(hipMemset(c->d_key[b], 0xff, sizeof(uint32_t) * c->N));
       rc |= hip_ok(hipMemset(c->d_claim, 0xff, sizeof(int) * c->N));
       rc |= lbm_particles_upload_belt2();
@@ -217,7 +282,8 @@ int lbm_destroy(lbm_ctx* c) {
   dfree(c->rho_r); dfree(c->rho_b); dfree(c->phi);
   dfree(c->d_status);
   dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
-  dfree(c->d_claim); dfree(c->d_tmpl); dfree(c->d_ball);
+  dfree(c->d_claim); dfree(c->d_tmpl); dfree(c->d_ball); dfree(c->d_ring); dfree(c->d_tlist);
+  dfree(c->d_nbr); dfree(c->d_nnbr);
   for (int e = 0; e < 2 * c->ev_cap; ++e) (void)hipEventDestroy(c->ev_pairs[e]);
   delete[] c->ev_pairs;
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -300,6 +366,11 @@ int lbm_set_particles(lbm_ctx* c, int32_t n, const double* pos) {
     dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
     c->d_myf = c->d_myf2 = c->d_myt = c->d_myt2 = nullptr;
     int rc = 0;
+
+    dfree(c->d_nbr); dfree(c->d_nnbr);
+    c->d_nbr = nullptr; c->d_nnbr = nullptr;
+    rc |= dalloc(&c->d_nbr, (size_t)lbm_particles_max_neighbours() * n);
+    rc |= dalloc(&c->d_nnbr, (size_t)n);
     for (int b = 0; b < 2; ++b) {
       rc |= dalloc(&c->d_pos[b], (size_t)13 * n);
       rc |= dalloc(&c->d_force[b], (size_t)12 * n);
@@ -351,12 +422,14 @@ int lbm_step(lbm_ctx* c, int32_t nsteps) {
       rc |= lbm_particles_flags(c, c->cur);
       rc |= lbm_particles_erase(c);
     }
-    rc |= lbm_launch_density(c);
+    const bool fused = c->variant == LBM_KERNEL_FUSED;
+    if (!fused) rc |= lbm_launch_density(c);
     if (parts) {
+      if (fused) rc |= lbm_particles_shell(c);
       rc |= lbm_particles_fill(c);
       rc |= lbm_particles_wetting(c);
     }
-    rc |= lbm_launch_collide(c);
+    rc |= fused ? lbm_launch_fused(c) : lbm_launch_collide(c);
     if (parts) {
       rc |= lbm_particles_bounce(c);
       rc |= lbm_particles_integrate(c);
@@ -392,6 +465,22 @@ int lbm_get_pops(lbm_ctx* c, int32_t color, float* out) {
 int lbm_get_rho(lbm_ctx* c, float* rr, float* rb) {
   if (!c) return LBM_ERR_ARG;
   int rc = 0;
+  if (c->variant == LBM_KERNEL_FUSED && c->step > 0) {
+    /* the single-pass step keeps densities only where particle kernels need
+       them: form the rest from the level the last step started from */
+    float *d_rr = nullptr, *d_rb = nullptr;
+    rc |= dalloc(&d_rr, c->N);
+    rc |= dalloc(&d_rb, c->N);
+    if (!rc) {
+      rc |= lbm_launch_rho_from_pops(c, d_rr, d_rb);
+      if (rr) rc |= hip_ok(hipMemcpyAsync(rr, d_rr, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+      if (rb) rc |= hip_ok(hipMemcpyAsync(rb, d_rb, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
+      rc |= hip_ok(hipStreamSynchronize(c->stream));
+    }
+    dfree(d_rr);
+    dfree(d_rb);
+    return rc;
+  }
   if (rr) rc |= hip_ok(hipMemcpyAsync(rr, c->rho_r, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
   if (rb) rc |= hip_ok(hipMemcpyAsync(rb, c->rho_b, sizeof(float) * c->N, hipMemcpyDeviceToHost, c->stream));
   rc |= hip_ok(hipStreamSynchronize(c->stream));
@@ -466,6 +555,7 @@ int lbm_status(lbm_ctx* c, int32_t* code, int32_t* step, char* msg, int32_t len)
     else if (st.code == LBM_ERR_BIG_FORCE) m = "nve_lf] Big force (kernels_fluid_PART.CUF:1532)";
     else if (st.code == LBM_ERR_EXTREME_VEL) m = "nve_lf] Extreme vel (kernels_fluid_PART.CUF:1545)";
     else if (st.code == LBM_ERR_OUT_OF_BOX) m = "nve] particle out of box (kernels_fluid_PART.CUF:1581)";
+    else if (st.code == LBM_ERR_LIST_OVERFLOW) m = "particle neighbour list full (cf. kernels_fluid_PART.CUF:2436)";
     std::snprintf(msg, (size_t)len, "%s", m);
   }
   return rc;
@@ -536,8 +626,143 @@ int lbm_get_kernel_timing(lbm_ctx* c, double* total_ms, int64_t* launches) {
 int lbm_kernel_info(lbm_ctx* c, char* name, int32_t len, double* bytes_per_lu) {
   if (!c) return LBM_ERR_ARG;
   const char* st = c->storage == LBM_STORE_F32 ? "f32" : (c->storage == LBM_STORE_F16 ? "f16" : "bf16");
-  if (name && len > 0) std::snprintf(name, (size_t)len, "k_collide_push<%s,%s>", st, c->np > 0 ? "part" : "fluid");
+  if (name && len > 0) {
+    if (c->variant == LBM_KERNEL_FUSED) {
+      int tx, ty, wx;
+      lbm_fused_tile_shape(c->tile, &tx, &ty, &wx);
+      std::snprintf(name, (size_t)len, "k_fused<%s,%s,%dx%d%s,zc=%d>", st, c->np > 0 ? "part" : "fluid", tx, ty,
+                    wx ? "w" : "", c->zchunk);
+    } else {
+      std::snprintf(name, (size_t)len, "k_collide_push<%s,%s>", st, c->np > 0 ? "part" : "fluid");
+    }
+  }
   if (bytes_per_lu) *bytes_per_lu = 2.0 * 19.0 * 2.0 * (double)c->esize + (c->np > 0 ? 1.0 : 0.0);
+  return LBM_OK;
+}
+
+}  /* extern "C" */
+
+extern "C" {
+
+int lbm_set_kernel(lbm_ctx* c, int32_t variant, int32_t tile, int32_t zchunk) {
+  if (!c) return LBM_ERR_ARG;
+  if (variant == LBM_KERNEL_TWOPASS) {
+    c->variant = variant;
+    return LBM_OK;
+  }
+  if (variant != LBM_KERNEL_FUSED) return LBM_ERR_ARG;
+  if (zchunk <= 0) zchunk = c->nz < 32 ? c->nz : 32;
+  if (zchunk > c->nz) zchunk = c->nz;
+  if (!lbm_fused_tile_valid(c, tile, zchunk)) return LBM_ERR_ARG;
+  c->variant = variant;
+  c->tile = tile;
+  c->zchunk = zchunk;
+  return LBM_OK;
+}
+
+int lbm_get_kernel(lbm_ctx* c, int32_t* variant, int32_t* tile, int32_t* zchunk) {
+  if (!c) return LBM_ERR_ARG;
+  if (variant) *variant = c->variant;
+  if (tile) *tile = c->tile;
+  if (zchunk) *zchunk = c->zchunk;
+  return LBM_OK;
+}
+
+int32_t lbm_num_tiles(void) { return lbm_fused_num_tiles(); }
+
+int lbm_tile_shape(int32_t tile, int32_t* tx, int32_t* ty, int32_t* wrap) {
+  if (tile < 0 || tile >= lbm_fused_num_tiles()) return LBM_ERR_ARG;
+  int a, b, w;
+  lbm_fused_tile_shape(tile, &a, &b, &w);
+  if (tx) *tx = a;
+  if (ty) *ty = b;
+  if (wrap) *wrap = w;
+  return LBM_OK;
+}
+
+int lbm_autotune(lbm_ctx* c, int32_t reps, int32_t* tile_out, int32_t* zc_out, double* best_ms) {
+  if (!c || reps <= 0) return LBM_ERR_ARG;
+  if (!c->initialised) return LBM_ERR_STATE;
+  /* the fused kernel only reads the current level, so repeated launches into
+     a scratch level leave the state as it was; the status word is restored */
+  void* scratch = nullptr;
+  if (hipMalloc(&scratch, c->esize * 38 * c->N) != hipSuccess) return LBM_ERR_ALLOC;
+  lbm::DevStatus saved;
+  int rc = hip_ok(hipMemcpyAsync(&saved, c->d_status, sizeof saved, hipMemcpyDeviceToHost, c->stream));
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  hipEvent_t e0, e1;
+  rc |= hip_ok(hipEventCreate(&e0));
+  rc |= hip_ok(hipEventCreate(&e1));
+  const int zcs[] = {8, 16, 32, 64, 128, 256};
+  double best = 1e30;
+  int bt = -1, bz = 0;
+  const bool parts = c->np > 0;
+  for (int t = 0; !rc && t < lbm_fused_num_tiles(); ++t) {
+    int last_zc = -1;
+    for (int zc0 : zcs) {
+      const int zc = zc0 > c->nz ? c->nz : zc0;
+      if (zc == last_zc) continue;
+      last_zc = zc;
+      if (!lbm_fused_tile_valid(c, t, zc)) continue;
+      rc |= lbm_launch_fused_with(c, t, zc, scratch);   /* warm-up */
+      rc |= hip_ok(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < reps; ++r) rc |= lbm_launch_fused_with(c, t, zc, scratch);
+      rc |= hip_ok(hipEventRecord(e1, c->stream));
+      rc |= hip_ok(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      rc |= hip_ok(hipEventElapsedTime(&ms, e0, e1));
+      if (rc) break;
+      const double per = (double)ms / reps;
+      if (per < best) { best = per; bt = t; bz = zc; }
+    }
+  }
+  (void)parts;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  rc |= hip_ok(hipMemcpyAsync(c->d_status, &saved, sizeof saved, hipMemcpyHostToDevice, c->stream));
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  (void)hipFree(scratch);
+  if (rc) return LBM_ERR_HIP;
+  if (bt < 0) return LBM_ERR_ARG;
+  c->variant = LBM_KERNEL_FUSED;
+  c->tile = bt;
+  c->zchunk = bz;
+  if (tile_out) *tile_out = bt;
+  if (zc_out) *zc_out = bz;
+  if (best_ms) *best_ms = best;
+  return LBM_OK;
+}
+
+}  /* extern "C" */
+
+extern "C" {
+
+int lbm_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
+  if (bytes < 1024 || reps <= 0 || !gbs) return LBM_ERR_ARG;
+  const size_t n4 = (size_t)bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n4 * 16) != hipSuccess) return LBM_ERR_ALLOC;
+  if (hipMalloc(&b, n4 * 16) != hipSuccess) { (void)hipFree(a); return LBM_ERR_ALLOC; }
+  hipStream_t st;
+  int rc = hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  rc |= hip_ok(hipEventCreate(&e0));
+  rc |= hip_ok(hipEventCreate(&e1));
+  rc |= hip_ok(hipMemsetAsync(a, 0, n4 * 16, st));
+  rc |= lbm_launch_copy_probe(a, b, n4, st);   /* warm-up */
+  rc |= hip_ok(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) rc |= lbm_launch_copy_probe(a, b, n4, st);
+  rc |= hip_ok(hipEventRecord(e1, st));
+  rc |= hip_ok(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  rc |= hip_ok(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return LBM_ERR_HIP;
+  *gbs = 2.0 * (double)(n4 * 16) * reps / ((double)ms * 1e-3) / 1e9;
   return LBM_OK;
 }
 

@@ -56,6 +56,19 @@ constexpr float P3D = (1.0f / 6.0f) * (1.0f / 6.0f) * (1.0f / 6.0f);
 constexpr float P0D = (2.0f / 3.0f) * (2.0f / 3.0f) * (2.0f / 3.0f);
 constexpr float A1D = P1D / CSSQ, A2D = P2D / CSSQ, A3D = P3D / CSSQ;
 
+/* D3Q27 links in the reference order (dimensions_m.CUF:275-282): 0..18 are the
+ * D3Q19 links, 19..26 the corners. */
+__host__ __device__ constexpr int EX27(int l) {
+  return l < 19 ? EX(l) : (l == 19 || l == 21 || l == 24 || l == 25 ? 1 : -1);
+}
+__host__ __device__ constexpr int EY27(int l) {
+  return l < 19 ? EY(l) : (l == 19 || l == 21 || l == 23 || l == 26 ? 1 : -1);
+}
+__host__ __device__ constexpr int EZ27(int l) {
+  return l < 19 ? EZ(l) : (l == 19 || l == 22 || l == 23 || l == 25 ? 1 : -1);
+}
+__host__ __device__ constexpr float A27(int l) { return l < 7 ? A1D : (l < 19 ? A2D : A3D); }
+
 /* pref_bouzidi = 2/cssq, dimensions_m.CUF:99 */
 constexpr float BUZ = 2.0f / CSSQ;
 
@@ -140,11 +153,21 @@ struct lbm_ctx {
   int* d_stamp[2];        /* (3, np) centres stamped into flags[b]             */
   uint32_t* d_key[2];     /* (N) owner key per time level                      */
   int* d_claim;           /* (N) make_fluid writer resolution                  */
+  int* d_nbr;             /* (32, np) neighbour lists of the link kernel        */
+  int* d_nnbr;            /* (np)                                               */
   int8_t* d_tmpl;         /* (4, ntmpl) template offsets + value               */
   int ntmpl;
+  int8_t* d_ring;         /* (4, nring) D3Q27 ring of the template's surface */
+  int nring;
+  int8_t* d_tlist;        /* (4, ntlist) surface (spherelist) template nodes */
+  int ntlist;
   int8_t* d_ball;         /* (4, nball) offsets with r^2 <= rmax^2; [3]=1 if <   */
   int nball;
   int rmax;
+  /* collide-stream kernel: LBM_KERNEL_*, fused tile index and z chunk */
+  int variant;
+  int tile;
+  int zchunk;
   /* timing: one event pair per collide-stream launch, read back lazily */
   hipEvent_t* ev_pairs;   /* 2 * ev_cap events                                 */
   int ev_cap;
@@ -161,6 +184,13 @@ void lbm_timing_end(lbm_ctx* c);
 /* launchers implemented in lbm_fluid.hip / lbm_particles.hip */
 int lbm_launch_density(lbm_ctx* c);
 int lbm_launch_collide(lbm_ctx* c);
+int lbm_launch_fused(lbm_ctx* c);
+int lbm_launch_fused_with(lbm_ctx* c, int tile, int zc, void* dst);
+int lbm_fused_num_tiles();
+bool lbm_fused_tile_valid(const lbm_ctx* c, int tile, int zc);
+void lbm_fused_tile_shape(int tile, int* tx, int* ty, int* wx);
+int lbm_launch_rho_from_pops(lbm_ctx* c, float* d_rr, float* d_rb);
+int lbm_launch_copy_probe(const void* a, void* b, size_t n_float4, hipStream_t st);
 int lbm_launch_init_eq(lbm_ctx* c, int type, const float u0[3], const float* d_vel);
 int lbm_launch_moments(lbm_ctx* c, float* d_rr, float* d_rb, float* d_vel, float* d_phase);
 int lbm_launch_sums(lbm_ctx* c, double* d_out);
@@ -171,5 +201,7 @@ int lbm_particles_flags(lbm_ctx* c, int level);
 int lbm_particles_erase(lbm_ctx* c);
 int lbm_particles_fill(lbm_ctx* c);
 int lbm_particles_wetting(lbm_ctx* c);
+int lbm_particles_shell(lbm_ctx* c);
+int lbm_particles_max_neighbours();
 int lbm_particles_bounce(lbm_ctx* c);
 int lbm_particles_integrate(lbm_ctx* c);

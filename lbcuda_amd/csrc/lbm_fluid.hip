@@ -186,23 +186,17 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
                     inv * (fb[3] - fb[4] + fb[7] - fb[8] + fb[9] - fb[10] + fb[15] - fb[16] - fb[17] + fb[18]);
     const float w = inv * (fr[5] - fr[6] + fr[11] - fr[12] + fr[13] - fr[14] + fr[15] - fr[16] + fr[17] - fr[18]) +
                     inv * (fb[5] - fb[6] + fb[11] - fb[12] + fb[13] - fb[14] + fb[15] - fb[16] + fb[17] - fb[18]);
-    /* 26-point D3Q27 gradient of phi, kernels_fluid_CG.CUF:720-741; terms with
-       a zero component are skipped (a NaN neighbour still makes |grad|^2 NaN) */
+    /* 26-point gradient in the D3Q27 order of kernels_fluid_CG.CUF:720-741;
+       zero-component terms add nothing and are skipped */
     float px = 0.0f, py = 0.0f, pz = 0.0f;
 #pragma unroll
-    for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-      for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int nz_ = (dx != 0) + (dy != 0) + (dz != 0);
-          if (nz_ == 0) continue;
-          const float wgt = nz_ == 1 ? A1D : (nz_ == 2 ? A2D : A3D);
-          const float ps = a.phi[nb(dx, dy, dz)];
-          if (dx) px = px + wgt * (float)dx * ps;
-          if (dy) py = py + wgt * (float)dy * ps;
-          if (dz) pz = pz + wgt * (float)dz * ps;
-        }
+    for (int l = 1; l < 27; ++l) {
+      const int dx = EX27(l), dy = EY27(l), dz = EZ27(l);
+      const float ps = a.phi[nb(dx, dy, dz)];
+      if (dx) px = px + A27(l) * (float)dx * ps;
+      if (dy) py = py + A27(l) * (float)dy * ps;
+      if (dz) pz = pz + A27(l) * (float)dz * ps;
+    }
     /* BGK towards f^eq(rho_k, u) per colour, :794-800 */
     const float usq = (0.5f / CSSQ) * (u * u + v * v + w * w);
 #pragma unroll
@@ -351,6 +345,33 @@ __global__ void k_sums_final(const double* part, int nblk, double* out) {
     for (int q = 0; q < 5; ++q) out[q] = sh[q][0];
 }
 
+/* densities as the reference's rhoR_d/rhoB_d hold them after a step of the
+ * single-pass kernel: sums of the populations the step started from on its
+ * fluid nodes, the stored (wetting / MINDENS) values elsewhere */
+template <class S>
+__global__ __launch_bounds__(256) void k_rho_from_pops(FluidArgs a, const uint8_t* fl, float* rr_o, float* rb_o) {
+  size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.N) return;
+  const S* src = (const S*)a.src;
+  float rr, rb;
+  if (fl[n] == FL_FLUID) {
+    rr = ldp(src, n, 0, a.rho0);
+    rb = ldp(src, (size_t)19 * a.N + n, 0, a.rho0);
+#pragma unroll
+    for (int l = 1; l < 19; ++l) {
+      rr = rr + ldp(src, (size_t)l * a.N + n, l, a.rho0);
+      rb = rb + ldp(src, (size_t)(19 + l) * a.N + n, l, a.rho0);
+    }
+    if (rr < 0.0f) rr = 0.0f;
+    if (rb < 0.0f) rb = 0.0f;
+  } else {
+    rr = a.rho_r[n];
+    rb = a.rho_b[n];
+  }
+  rr_o[n] = rr;
+  rb_o[n] = rb;
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_unpack(FluidArgs a, int color, float* out) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,6 +387,13 @@ __global__ __launch_bounds__(256) void k_pack(FluidArgs a, const float* pr, cons
   int l = (int)(i / a.N);
   stp((S*)a.dst, i, l, pr[i], a.rho0);
   stp((S*)a.dst, (size_t)19 * a.N + i, l, pb[i], a.rho0);
+}
+
+/* plain device copy (read n float4, write n float4): the measured HBM
+ * ceiling the fluid kernels are compared with */
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
 }
 
 FluidArgs make_args(lbm_ctx* c) {
@@ -448,6 +476,15 @@ int lbm_launch_sums(lbm_ctx* c, double* d_out) {
   return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
 }
 
+int lbm_launch_rho_from_pops(lbm_ctx* c, float* d_rr, float* d_rb) {
+  FluidArgs a = make_args(c);
+  a.src = c->pop[1 - c->cur];
+  LBM_STORAGE_SWITCH(c->storage,
+    hipLaunchKernelGGL(k_rho_from_pops<S>, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream, a,
+                       c->flags[1 - c->cur], d_rr, d_rb))
+  return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
+}
+
 int lbm_launch_unpack_pops(lbm_ctx* c, int color, float* d_out) {
   FluidArgs a = make_args(c);
   LBM_STORAGE_SWITCH(c->storage,
@@ -460,5 +497,11 @@ int lbm_launch_pack_pops(lbm_ctx* c, const float* pr, const float* pb) {
   a.dst = c->pop[c->cur];
   LBM_STORAGE_SWITCH(c->storage,
     hipLaunchKernelGGL(k_pack<S>, dim3(nblocks(c->N * 19, 256)), dim3(256), 0, c->stream, a, pr, pb))
+  return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
+}
+
+int lbm_launch_copy_probe(const void* a, void* b, size_t n_float4, hipStream_t st) {
+  const unsigned grid = 256 * 64;
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, (const float4*)a, (float4*)b, n_float4);
   return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
 }

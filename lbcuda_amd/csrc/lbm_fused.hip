@@ -1,0 +1,394 @@
+/*
+ * lbm_fused.hip -- single-pass colour-gradient step for gfx950.
+ *
+ * One kernel does what the reference spreads over init_rhoR_CG/init_rhoB_CG,
+ * the periodic refreshes, time_step_CG and the push streaming
+ * (kernels_fluid_CG.CUF:398-509, 531-848, 866-1141; main.CUF:142-220):
+ * densities, phase field, 26-point gradient, BGK + perturbation +
+ * recolouring, and push streaming into the other time level.
+ *
+ * Work decomposition.  A workgroup owns a TX x TY column of the box over a
+ * chunk of z planes and marches up the chunk.  For plane z+1 it loads the
+ * populations of its own TX x TY nodes into registers and sums them into
+ * rhoR, rhoB and phi; it also sums the populations of the one-node ring of
+ * halo nodes around the column (their phi is needed by the gradient, their
+ * populations are not).  phi of planes z-1, z, z+1 lives in a three-plane
+ * ring in LDS, so the 26-point gradient of plane z reads LDS only.  Then
+ * plane z collides from registers and pushes.  When TX equals nx the x
+ * direction wraps inside the workgroup and there is no x halo at all.
+ *
+ * HBM traffic per interior node: 2 colours x 19 populations read once and
+ * written once (the density pre-pass of a two-kernel step is gone); the halo
+ * ring and the two extra planes of each z chunk are re-reads that neighbour
+ * workgroups fetch too, so they are served by L2 / MALL.  Consecutive
+ * workgroup ids are dealt round-robin over the 8 XCDs, so tiles are renumbered
+ * to give each XCD a contiguous range of neighbouring columns (shared halos
+ * stay in one L2).
+ *
+ * With particles the bounce-back copies of surface links are not done here:
+ * fluid nodes push as usual and non-fluid nodes carry their stored values,
+ * and the particle link kernel then writes every bounced slot (copy plus
+ * moving-wall term) once, so this kernel needs no neighbour-type tests.
+ *
+ * Non-fluid nodes (particle surface and interior) take rhoR/rhoB from the
+ * density arrays, which the particle kernels fill before this kernel runs
+ * (zero inside particles, wetting values on the surface), as init_rho*_CG and
+ * compute_*_rho leave them in the reference.
+ */
+#include "lbm_common.h"
+
+#include <math.h>
+
+using namespace lbm;
+
+namespace {
+
+/* Population access with a uniform per-direction base pointer and a 32-bit
+ * byte offset per lane (node index < 2^30), so loads and stores use the
+ * SGPR-base + VGPR-offset address form instead of 64-bit VGPR addresses. */
+template <class S>
+__device__ __forceinline__ float ldq(const S* base, uint32_t node, float shift) {
+  return Store<S>::ld((const S*)((const char*)base + (size_t)(node * (uint32_t)sizeof(S))), 0, shift);
+}
+template <class S>
+__device__ __forceinline__ void stq(S* base, uint32_t node, float v, float shift) {
+  Store<S>::st((S*)((char*)base + (size_t)(node * (uint32_t)sizeof(S))), 0, v, shift);
+}
+template <class S>
+__device__ __forceinline__ void cpq(S* dbase, uint32_t dnode, const S* sbase, uint32_t snode) {
+  *(S*)((char*)dbase + (size_t)(dnode * (uint32_t)sizeof(S))) =
+      *(const S*)((const char*)sbase + (size_t)(snode * (uint32_t)sizeof(S)));
+}
+
+/* rho = f_0 + f_1 + ... + f_18 in this order (init_rhoR_CG, :427-436) */
+template <class S>
+__device__ __forceinline__ void load_sum(const S* src, uint32_t n, size_t N, float rho0, float (&fr)[19],
+                                         float (&fb)[19], float& rr, float& rb) {
+#pragma unroll
+  for (int l = 0; l < 19; ++l) {
+    fr[l] = ldq(src + (size_t)l * N, n, W(l) * rho0);
+    fb[l] = ldq(src + (size_t)(19 + l) * N, n, W(l) * rho0);
+  }
+  rr = fr[0];
+  rb = fb[0];
+#pragma unroll
+  for (int l = 1; l < 19; ++l) {
+    rr = rr + fr[l];
+    rb = rb + fb[l];
+  }
+}
+
+template <class S>
+__device__ __forceinline__ void sum_only(const S* src, uint32_t n, size_t N, float rho0, float& rr, float& rb) {
+  float fr[19], fb[19];
+  load_sum(src, n, N, rho0, fr, fb, rr, rb);
+}
+
+template <int TX, int TY, bool WX>
+struct TileGeom {
+  static constexpr int NT = TX * TY;
+  static constexpr int LW = WX ? TX : TX + 2;   /* LDS row length */
+  static constexpr int MAXHALO = WX ? 2 * TX : 2 * (TX + 2) + 2 * TY;
+  static_assert(MAXHALO <= NT, "one halo node per thread at most");
+  __device__ static constexpr int col(int c) { return WX ? (c < 0 ? c + TX : (c >= TX ? c - TX : c)) : c + 1; }
+};
+
+template <class S, bool PART, int TX, int TY, bool WX>
+__global__ __launch_bounds__(TX * TY) void k_fused(FluidArgs a, int ntx, int nty, int nzc, int zc) {
+  using G = TileGeom<TX, TY, WX>;
+  __shared__ float sphi[3][TY + 2][G::LW];
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const size_t N = a.N;
+  const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
+  const float rho0 = a.rho0;
+  const S* src = (const S*)a.src;
+  S* dst = (S*)a.dst;
+
+  /* tile id: XCD k (= blockIdx.x mod 8) gets the contiguous tiles [k q, (k+1) q) */
+  const int nb = ntx * nty * nzc;
+  int t = blockIdx.x;
+  {
+    const int q = nb >> 3;
+    if (t < (q << 3)) t = (t & 7) * q + (t >> 3);
+  }
+  const int itx = t % ntx;
+  const int ity = (t / ntx) % nty;
+  const int izc = t / (ntx * nty);
+  const int x0 = itx * TX, y0 = ity * TY, z0 = izc * zc;
+  const int z1 = min(z0 + zc, nz);
+  const int w = WX ? TX : min(TX, nx - x0);
+  const int h = min(TY, ny - y0);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = tx + TX * ty;
+  const bool inside = tx < w && ty < h;
+  const int x = x0 + tx, y = y0 + ty;
+  const int cm = G::col(tx - 1), c0 = G::col(tx), cp = G::col(tx + 1);
+  const uint32_t nxy = inside ? (uint32_t)x + (uint32_t)y * (uint32_t)nx : 0u;
+
+  /* the halo node of this thread (fixed in x, y) */
+  const int nhalo = WX ? 2 * TX : 2 * (w + 2) + 2 * h;
+  const bool has_halo = tid < nhalo;
+  int hrow = 0, hcol = 0;
+  uint32_t hxy = 0;
+  if (has_halo) {
+    int c, r;
+    if (WX) {
+      r = tid < TX ? -1 : h;
+      c = tid < TX ? tid : tid - TX;
+    } else {
+      const int W2 = w + 2;
+      if (tid < W2) { r = -1; c = tid - 1; }
+      else if (tid < 2 * W2) { r = h; c = tid - W2 - 1; }
+      else { const int k2 = tid - 2 * W2; r = k2 >> 1; c = (k2 & 1) ? w : -1; }
+    }
+    hxy = (uint32_t)wrapc(x0 + c, nx) + (uint32_t)wrapc(y0 + r, ny) * (uint32_t)nx;
+    hrow = r + 1;
+    hcol = G::col(c);
+  }
+  /* x and y neighbours of the interior node */
+  const uint32_t xs[3] = {(uint32_t)(x == 0 ? nx - 1 : x - 1), (uint32_t)x, (uint32_t)(x == nx - 1 ? 0 : x + 1)};
+  const uint32_t ys[3] = {(uint32_t)(y == 0 ? ny - 1 : y - 1) * (uint32_t)nx, (uint32_t)y * (uint32_t)nx,
+                          (uint32_t)(y == ny - 1 ? 0 : y + 1) * (uint32_t)nx};
+
+  /* state of the interior node on the next plane (kept in registers) */
+  float qr[19], qb[19];
+  float qR = 0.0f, qB = 0.0f;
+  uint8_t qF = FL_FLUID;
+
+  /* phi of plane zl (wrapped) into ring slot s; the halo node first, then the
+   * interior node, whose populations are kept in qr/qb when keep is set.
+   * With particles the node type and the stored phase field (zero densities
+   * inside particles, wetting values on their surface) are loaded beside the
+   * populations, not before them, so no load waits on another. */
+  auto plane = [&](int zl, int s, bool keep) {
+    const int zg = zl < 0 ? zl + nz : (zl >= nz ? zl - nz : zl);
+    const uint32_t zoff = (uint32_t)zg * sxy;
+    if (has_halo) {
+      const uint32_t n = hxy + zoff;
+      const uint8_t hf = PART ? a.flags[n] : FL_FLUID;
+      const float hp = PART ? a.phi[n] : 0.0f;
+      float rr, rb;
+      sum_only(src, n, N, rho0, rr, rb);
+      if (rr < 0.0f) rr = 0.0f;
+      if (rb < 0.0f) rb = 0.0f;
+      sphi[s][hrow][hcol] = hf == FL_FLUID ? (rr - rb) / (rr + rb) : hp;
+    }
+    if (inside) {
+      const uint32_t n = nxy + zoff;
+      qF = PART ? a.flags[n] : FL_FLUID;
+      const float qp = PART ? a.phi[n] : 0.0f;
+      float rr, rb;
+      if (keep) load_sum(src, n, N, rho0, qr, qb, rr, rb);
+      else sum_only(src, n, N, rho0, rr, rb);
+      float ph = qp;
+      if (qF == FL_FLUID) {
+        if (rr < a.min_rho || rr > a.max_rho || rb < a.min_rho || rb > a.max_rho)
+          raise_error(a.status, LBM_ERR_RHO_RANGE, a.step);
+        if (rr < 0.0f) rr = 0.0f;
+        if (rb < 0.0f) rb = 0.0f;
+        ph = (rr - rb) / (rr + rb);
+      }
+      qR = rr;
+      qB = rb;
+      sphi[s][ty + 1][c0] = ph;
+    }
+  };
+
+  plane(z0 - 1, 0, false);
+  plane(z0, 1, true);
+  int sm = 0, s0 = 1, sp = 2;
+  for (int z = z0; z < z1; ++z) {
+    float fr[19], fb[19];
+#pragma unroll
+    for (int l = 0; l < 19; ++l) { fr[l] = qr[l]; fb[l] = qb[l]; }
+    const float rR = qR, rB = qB;
+    const uint8_t f = qF;
+    plane(z + 1, sp, z + 1 < z1);
+    __syncthreads();
+    if (inside) {
+      const uint32_t zs[3] = {(uint32_t)(z == 0 ? nz - 1 : z - 1) * sxy, (uint32_t)z * sxy,
+                              (uint32_t)(z == nz - 1 ? 0 : z + 1) * sxy};
+      const uint32_t n = xs[1] + ys[1] + zs[1];
+      auto nb3 = [&](int dx, int dy, int dz) -> uint32_t { return xs[dx + 1] + ys[dy + 1] + zs[dz + 1]; };
+      if (f == FL_FLUID) {
+        const float rs = rR + rB;
+        const float phf = sphi[s0][ty + 1][c0];
+        const float visc = 1.0f / (0.5f * (1.0f + phf) * a.inv_visc_r + 0.5f * (1.0f - phf) * a.inv_visc_b);
+        const float omega = 1.0f / (visc / CSSQ + 0.5f);
+        const float omm = 1.0f - omega;
+        const float inv = 1.0f / rs;
+        const float u = inv * (fr[1] - fr[2] + fr[7] - fr[8] - fr[9] + fr[10] + fr[11] - fr[12] - fr[13] + fr[14]) +
+                        inv * (fb[1] - fb[2] + fb[7] - fb[8] - fb[9] + fb[10] + fb[11] - fb[12] - fb[13] + fb[14]);
+        const float v = inv * (fr[3] - fr[4] + fr[7] - fr[8] + fr[9] - fr[10] + fr[15] - fr[16] - fr[17] + fr[18]) +
+                        inv * (fb[3] - fb[4] + fb[7] - fb[8] + fb[9] - fb[10] + fb[15] - fb[16] - fb[17] + fb[18]);
+        const float wv = inv * (fr[5] - fr[6] + fr[11] - fr[12] + fr[13] - fr[14] + fr[15] - fr[16] + fr[17] - fr[18]) +
+                         inv * (fb[5] - fb[6] + fb[11] - fb[12] + fb[13] - fb[14] + fb[15] - fb[16] + fb[17] - fb[18]);
+        /* 26-point gradient in the D3Q27 order of kernels_fluid_CG.CUF:720-741;
+           zero-component terms add nothing and are skipped */
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll
+        for (int l = 1; l < 27; ++l) {
+          const int dx = EX27(l), dy = EY27(l), dz = EZ27(l);
+          const int slot = dz < 0 ? sm : (dz > 0 ? sp : s0);
+          const int cc = dx < 0 ? cm : (dx > 0 ? cp : c0);
+          const float ps = sphi[slot][ty + 1 + dy][cc];
+          if (dx) px = px + A27(l) * (float)dx * ps;
+          if (dy) py = py + A27(l) * (float)dy * ps;
+          if (dz) pz = pz + A27(l) * (float)dz * ps;
+        }
+        /* BGK towards f^eq(rho_k, u) per colour, :794-800 */
+        const float usq = (0.5f / CSSQ) * (u * u + v * v + wv * wv);
+#pragma unroll
+        for (int l = 0; l < 19; ++l) {
+          const float uv = (1.0f / CSSQ) * (u * (float)EX(l) + v * (float)EY(l) + wv * (float)EZ(l));
+          const float brk = 1.0f + uv + 0.5f * (uv * uv) - usq;
+          fr[l] = fr[l] * omm + rR * W(l) * brk * omega;
+          fb[l] = fb[l] * omm + rB * W(l) * brk * omega;
+        }
+        const float pn2 = px * px + py * py + pz * pz;
+        if (pn2 > 1.e-20f) {
+          /* perturbation (:803-815) and recolouring (:817-837) */
+          const float pn = sqrtf(pn2);
+          const float ipn2 = 1.0f / pn2;
+          const float ipn = 1.0f / pn;
+          const float ac = 9.0f / 4.0f * omega * a.sigma;
+          const float bcoef = a.beta * rR * rB / (rs * rs);
+#pragma unroll
+          for (int l = 0; l < 19; ++l) {
+            const float ed = (float)EX(l) * px + (float)EY(l) * py + (float)EZ(l) * pz;
+            const float tt0 = pn * (W(l) * (ed * ed) * ipn2 - BL(l));
+            const float fd = fr[l] + fb[l] + ac * tt0;
+            const float cph = ed * INV_REC(l) * ipn;
+            const float tt = bcoef * cph;
+            const float feq = rR * W(l) + rB * W(l);
+            fr[l] = fd * rR * inv + tt * feq;
+            fb[l] = fd * rB * inv - tt * feq;
+          }
+        }
+      }
+      /* push: every node but spheredead sends its 18 moving populations to
+         x + e_l and keeps the rest population (flipflop*Pop0_CG, stream*_CG,
+         kernels_fluid_CG.CUF:866-1015); non-fluid nodes send their stored
+         values unchanged (no collision there, :689).  The slots a bounce-back
+         link claims are overwritten afterwards by the particle link kernel.
+         Stored values of non-fluid nodes are re-stored from the registers
+         they were loaded into: exact for float storage; for 16-bit storage
+         the float round trip (h + s) - s rounds back to h whenever
+         |h + s| < 0.5, which the shift s = w_l rho0 with rho0 <= 1.2
+         (checked at create) and stored deviations |h| < 0.1 satisfy. */
+      stq(dst, n, fr[0], W(0) * rho0);
+      stq(dst + (size_t)19 * N, n, fb[0], W(0) * rho0);
+      if (!PART || f != FL_DEAD) {
+#pragma unroll
+        for (int l = 1; l < 19; ++l) {
+          const uint32_t tn = nb3(EX(l), EY(l), EZ(l));
+          stq(dst + (size_t)l * N, tn, fr[l], W(l) * rho0);
+          stq(dst + (size_t)(19 + l) * N, tn, fb[l], W(l) * rho0);
+        }
+      }
+    }
+    __syncthreads();
+    const int so = sm;
+    sm = s0;
+    s0 = sp;
+    sp = so;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* tile table: the autotuner picks one entry and a z-chunk length */
+struct TileEntry {
+  int tx, ty;
+  bool wx;
+};
+constexpr TileEntry kTiles[] = {
+    {64, 4, false}, {64, 8, false}, {64, 16, false}, {128, 4, false}, {128, 8, false},
+    {256, 2, true}, {256, 4, true}, {128, 4, true},  {128, 8, true},  {64, 8, true},
+};
+constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
+
+FluidArgs make_fargs(lbm_ctx* c) {
+  FluidArgs a;
+  a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
+  a.N = c->N;
+  a.src = c->pop[c->cur];
+  a.dst = c->pop[1 - c->cur];
+  a.flags = c->flags[c->cur];
+  a.rho_r = c->rho_r; a.rho_b = c->rho_b; a.phi = c->phi;
+  const float viscR = CSSQ * (c->cfg.tau_r - 0.5f);
+  const float viscB = CSSQ * (c->cfg.tau_b - 0.5f);
+  a.inv_visc_r = 1.0f / viscR;
+  a.inv_visc_b = 1.0f / viscB;
+  a.sigma = c->cfg.sigma; a.beta = c->cfg.beta;
+  a.rho0 = c->storage == LBM_STORE_F32 ? 0.0f : c->cfg.store_rho0;
+  a.min_rho = c->cfg.min_rho; a.max_rho = c->cfg.max_rho;
+  a.status = c->d_status;
+  a.step = (int)c->step;
+  return a;
+}
+
+template <class S, bool PART, int TX, int TY, bool WX>
+void launch_one(const FluidArgs& a, int zc, hipStream_t st) {
+  const int ntx = (a.nx + TX - 1) / TX, nty = (a.ny + TY - 1) / TY, nzc = (a.nz + zc - 1) / zc;
+  hipLaunchKernelGGL((k_fused<S, PART, TX, TY, WX>), dim3((unsigned)(ntx * nty * nzc)), dim3(TX, TY), 0, st, a,
+                     ntx, nty, nzc, zc);
+}
+
+template <class S, bool PART>
+void launch_tile(int tile, const FluidArgs& a, int zc, hipStream_t st) {
+  switch (tile) {
+    case 0: launch_one<S, PART, 64, 4, false>(a, zc, st); break;
+    case 1: launch_one<S, PART, 64, 8, false>(a, zc, st); break;
+    case 2: launch_one<S, PART, 64, 16, false>(a, zc, st); break;
+    case 3: launch_one<S, PART, 128, 4, false>(a, zc, st); break;
+    case 4: launch_one<S, PART, 128, 8, false>(a, zc, st); break;
+    case 5: launch_one<S, PART, 256, 2, true>(a, zc, st); break;
+    case 6: launch_one<S, PART, 256, 4, true>(a, zc, st); break;
+    case 7: launch_one<S, PART, 128, 4, true>(a, zc, st); break;
+    case 8: launch_one<S, PART, 128, 8, true>(a, zc, st); break;
+    default: launch_one<S, PART, 64, 8, true>(a, zc, st); break;
+  }
+}
+
+}  // namespace
+
+int lbm_fused_num_tiles() { return kNumTiles; }
+
+bool lbm_fused_tile_valid(const lbm_ctx* c, int tile, int zc) {
+  if (tile < 0 || tile >= kNumTiles || zc < 1) return false;
+  const TileEntry& e = kTiles[tile];
+  if (e.wx && c->nx != e.tx) return false;
+  if (!e.wx && c->nx < 3) return false;
+  return true;
+}
+
+void lbm_fused_tile_shape(int tile, int* tx, int* ty, int* wx) {
+  const TileEntry& e = kTiles[tile < 0 || tile >= kNumTiles ? 0 : tile];
+  *tx = e.tx; *ty = e.ty; *wx = e.wx ? 1 : 0;
+}
+
+/* launch the fused step with the given tile / z chunk; dst overrides the
+ * destination buffer (the autotuner times into a scratch buffer) */
+int lbm_launch_fused_with(lbm_ctx* c, int tile, int zc, void* dst) {
+  if (!lbm_fused_tile_valid(c, tile, zc)) return LBM_ERR_ARG;
+  FluidArgs a = make_fargs(c);
+  if (dst) a.dst = dst;
+  const bool part = c->np > 0;
+#define LBM_FUSED_GO(S)                                       \
+  if (part) launch_tile<S, true>(tile, a, zc, c->stream);     \
+  else launch_tile<S, false>(tile, a, zc, c->stream);
+  switch (c->storage) {
+    case LBM_STORE_F32: { LBM_FUSED_GO(float) } break;
+    case LBM_STORE_F16: { LBM_FUSED_GO(__half) } break;
+    default: { LBM_FUSED_GO(__hip_bfloat16) } break;
+  }
+#undef LBM_FUSED_GO
+  return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
+}
+
+int lbm_launch_fused(lbm_ctx* c) {
+  lbm_timing_begin(c);
+  int rc = lbm_launch_fused_with(c, c->tile, c->zchunk, nullptr);
+  lbm_timing_end(c);
+  return rc;
+}

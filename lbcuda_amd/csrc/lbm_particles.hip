@@ -23,6 +23,7 @@ using namespace lbm;
 namespace {
 
 constexpr int TPB = 256;
+constexpr int MAXNB = 32;   /* neighbour-list capacity per particle */
 
 /* D3Q27 links in the reference order (dimensions_m.CUF:275-282) */
 __constant__ int8_t C_E27[27][3] = {
@@ -57,6 +58,14 @@ struct PartArgs {
   int* claim;
   const int8_t* tmpl;      /* (4, ntmpl)                         */
   int ntmpl;
+  const int8_t* ring;      /* (4, nring) D3Q27 ring of the surface */
+  int nring;
+  const int8_t* tlist;     /* (4, ntlist) surface nodes of the template */
+  int ntlist;
+  int* nbr;                /* (MAXNB, np) particles whose ball can reach
+                              this particle's template, ascending       */
+  int* nnbr;               /* (np) number found                         */
+  int rmax;
   const int8_t* ball;      /* (4, nball), [3] = 1 if r^2 < rmax^2 */
   int nball;
   float* rho_r;
@@ -68,6 +77,9 @@ struct PartArgs {
   float radius;
   int lrotate;
   int wetting;
+  int rho_from_pops;       /* single-pass step: densities away from the
+                              particle shells are not stored, sum them    */
+  int lag_from_pops;       /* erase: lagging density from the other level */
   DevStatus* status;
   int step;
 };
@@ -76,11 +88,10 @@ __device__ __forceinline__ long nint(double x) { return lround(x); }   /* Fortra
 
 __device__ __forceinline__ size_t node_at(const PartArgs& a, int cx, int cy, int cz, int dx, int dy, int dz,
                                           int* gx = nullptr, int* gy = nullptr, int* gz = nullptr) {
-  /* centres are 1-based lattice coordinates, nodes 0-based */
-  int x = cx - 1 + dx, y = cy - 1 + dy, z = cz - 1 + dz;
-  x = ((x % a.nx) + a.nx) % a.nx;
-  y = ((y % a.ny) + a.ny) % a.ny;
-  z = ((z % a.nz) + a.nz) % a.nz;
+  /* centres are 1-based lattice coordinates, nodes 0-based; centres lie in
+     [1, n+1] and offsets within rmax + 2 <= n (checked at create), so one
+     conditional wrap per axis suffices */
+  int x = wrapc(cx - 1 + dx, a.nx), y = wrapc(cy - 1 + dy, a.ny), z = wrapc(cz - 1 + dz, a.nz);
   if (gx) { *gx = x; *gy = y; *gz = z; }
   return (size_t)x + (size_t)a.nx * ((size_t)y + (size_t)a.ny * z);
 }
@@ -177,31 +188,29 @@ __device__ __forceinline__ void stp(void* p, size_t i, int l, float v, float rho
   Store<S>::st((S*)p, i, v, W(l) * rho0);
 }
 
-/* atomic subtraction on a stored population (used by the moving-wall term) */
-__device__ __forceinline__ void atomic_sub_pop(float* p, size_t i, float v, float) { atomicAdd(p + i, -v); }
+/* rho_k = f_0 + ... + f_18 (init_rhoR_CG order, :427-436), clamped at
+ * MINDENS = 0 (:443-446) */
 template <class S>
-__device__ void atomic_sub_pop16(S* p, size_t i, float v, float shift) {
-  /* 16-bit value inside an aligned 32-bit word: CAS loop */
-  uintptr_t addr = (uintptr_t)(p + i);
-  unsigned int* w = (unsigned int*)(addr & ~(uintptr_t)3);
-  const int hi = (addr & 2) ? 1 : 0;
-  unsigned int old = *w, assumed;
-  do {
-    assumed = old;
-    unsigned short h = hi ? (unsigned short)(assumed >> 16) : (unsigned short)(assumed & 0xffffu);
-    S sv;
-    __builtin_memcpy(&sv, &h, 2);
-    float f = Store<S>::ld(&sv, 0, shift) - v;
-    Store<S>::st(&sv, 0, f, shift);
-    __builtin_memcpy(&h, &sv, 2);
-    unsigned int nw = hi ? ((assumed & 0x0000ffffu) | ((unsigned int)h << 16)) : ((assumed & 0xffff0000u) | h);
-    old = atomicCAS(w, assumed, nw);
-  } while (old != assumed);
+__device__ void rho_sum(const void* pops, size_t N, size_t n, float rho0, float* rr, float* rb) {
+  float r = ldp<S>(pops, n, 0, rho0), b = ldp<S>(pops, (size_t)19 * N + n, 0, rho0);
+#pragma unroll
+  for (int l = 1; l < 19; ++l) {
+    r = r + ldp<S>(pops, (size_t)l * N + n, l, rho0);
+    b = b + ldp<S>(pops, (size_t)(19 + l) * N + n, l, rho0);
+  }
+  *rr = r < 0.0f ? 0.0f : r;
+  *rb = b < 0.0f ? 0.0f : b;
 }
+
+/* density of a node that is fluid on both levels (belt averages) */
 template <class S>
-__device__ __forceinline__ void atomic_sub(void* p, size_t i, int l, float v, float rho0) {
-  if constexpr (sizeof(S) == 4) atomic_sub_pop((float*)p, i, v, 0.0f);
-  else atomic_sub_pop16<S>((S*)p, i, v, W(l) * rho0);
+__device__ __forceinline__ void belt_rho(const PartArgs& a, size_t m, float* rr, float* rb) {
+  if (a.rho_from_pops) {
+    rho_sum<S>(a.pop_src, a.N, m, a.rho0, rr, rb);
+  } else {
+    *rr = a.rho_r[m];
+    *rb = a.rho_b[m];
+  }
 }
 
 /* ---------------- flags ------------------------------------------------ */
@@ -255,7 +264,16 @@ __global__ __launch_bounds__(TPB) void k_erase(PartArgs a) {
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
     const uint8_t fnew = a.fl_cur[n];
     if (a.fl_old[n] != FL_FLUID || !(fnew == FL_LIST || fnew == FL_DEAD)) continue;
-    const float rR = a.rho_r[n], rB = a.rho_b[n];
+    /* rhoR_d/rhoB_d still hold the previous step's densities here (del_fluid
+       runs before init_rho*_CG, main.CUF:130-144); the single-pass step does
+       not store them, so they are summed from the previous level */
+    float rR, rB;
+    if (a.lag_from_pops) {
+      rho_sum<S>(a.pop_dst, a.N, n, a.rho0, &rR, &rB);
+    } else {
+      rR = a.rho_r[n];
+      rB = a.rho_b[n];
+    }
     const float inv = 1.0f / (rR + rB);
     float fr[19], fb[19];
 #pragma unroll
@@ -322,8 +340,10 @@ __global__ __launch_bounds__(TPB) void k_fill_apply(PartArgs a) {
     for (int l = 1; l < 27; ++l) {
       size_t m = wrap_node(a, gx + C_E27[l][0], gy + C_E27[l][1], gz + C_E27[l][2]);
       if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
-        Rs = Rs + a.rho_r[m];
-        Bs = Bs + a.rho_b[m];
+        float mr, mb;
+        belt_rho<S>(a, m, &mr, &mb);
+        Rs = Rs + mr;
+        Bs = Bs + mb;
         D = D + 1;
       }
     }
@@ -332,8 +352,10 @@ __global__ __launch_bounds__(TPB) void k_fill_apply(PartArgs a) {
       for (int l = 0; l < 98; ++l) {
         size_t m = wrap_node(a, gx + C_BELT2[l][0], gy + C_BELT2[l][1], gz + C_BELT2[l][2]);
         if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
-          Rs = Rs + a.rho_r[m];
-          Bs = Bs + a.rho_b[m];
+          float mr, mb;
+          belt_rho<S>(a, m, &mr, &mb);
+          Rs = Rs + mr;
+          Bs = Bs + mb;
           D = D + 1;
         }
       }
@@ -382,14 +404,14 @@ __global__ __launch_bounds__(TPB) void k_fill_release(PartArgs a) {
 }
 
 /* ---------------- wetting of surface nodes -------------------------------- */
+template <class S>
 __global__ __launch_bounds__(TPB) void k_wetting(PartArgs a) {
   const int p = blockIdx.x;
   int c[3];
   centre(a.pos_cur, p, c);
-  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
-    if (a.tmpl[4 * t + 3] != FL_LIST) continue;
+  for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
     int gx, gy, gz;
-    size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2], &gx, &gy, &gz);
+    size_t n = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
     if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) continue;
     float rr, rb;
     if (a.wetting == LBM_WET_JANUS) {
@@ -408,13 +430,23 @@ __global__ __launch_bounds__(TPB) void k_wetting(PartArgs a) {
       const double ac = ax * r[0] + ay * r[1] + az * r[2];
       if (ac < 0.0) { rr = 0.0f; rb = 1.0f; } else { rr = 1.0f; rb = 0.0f; }
     } else {
-      /* compute_densities_wall + compute_onebelt_density_2fl_weights, PART:267-297, 864-894 */
-      float Rs = 0.0f, Bs = 0.0f, Ds = 0.0f;
+      /* compute_densities_wall + compute_onebelt_density_2fl_weights, PART:267-297, 864-894;
+         the 26 neighbours' types and densities are loaded in one batch */
+      uint8_t f27[27];
+      float r27[27], b27[27];
+#pragma unroll
       for (int l = 1; l < 27; ++l) {
-        size_t m = wrap_node(a, gx + C_E27[l][0], gy + C_E27[l][1], gz + C_E27[l][2]);
-        if (a.fl_cur[m] == FL_FLUID) {
-          Rs = Rs + PD27(l) * a.rho_r[m];
-          Bs = Bs + PD27(l) * a.rho_b[m];
+        const size_t m = wrap_node(a, gx + EX27(l), gy + EY27(l), gz + EZ27(l));
+        f27[l] = a.fl_cur[m];
+        r27[l] = a.rho_r[m];
+        b27[l] = a.rho_b[m];
+      }
+      float Rs = 0.0f, Bs = 0.0f, Ds = 0.0f;
+#pragma unroll
+      for (int l = 1; l < 27; ++l) {
+        if (f27[l] == FL_FLUID) {
+          Rs = Rs + PD27(l) * r27[l];
+          Bs = Bs + PD27(l) * b27[l];
           Ds = Ds + PD27(l);
         }
       }
@@ -427,8 +459,10 @@ __global__ __launch_bounds__(TPB) void k_wetting(PartArgs a) {
         for (int l = 0; l < 98; ++l) {
           size_t m = wrap_node(a, gx + C_BELT2[l][0], gy + C_BELT2[l][1], gz + C_BELT2[l][2]);
           if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
-            R2 = R2 + a.rho_r[m];
-            B2 = B2 + a.rho_b[m];
+            float mr, mb;
+            belt_rho<S>(a, m, &mr, &mb);
+            R2 = R2 + mr;
+            B2 = B2 + mb;
             D = D + 1;
           }
         }
@@ -443,14 +477,152 @@ __global__ __launch_bounds__(TPB) void k_wetting(PartArgs a) {
   }
 }
 
+/* ---------------- densities around each particle (single-pass step) -----
+ * The single-pass fluid kernel sums densities of fluid nodes itself and reads
+ * rhoR_d/rhoB_d only on non-fluid nodes.  The particle kernels that follow
+ * the density pass in the reference (make_fluid, compute_*_rho, partBB_CG*)
+ * read densities of fluid nodes next to particle surfaces, so this kernel
+ * forms them there from the populations of this step, and gives the
+ * particle's own non-fluid nodes MINDENS = 0 (init_rhoR_CG / init_rhoB_CG,
+ * kernels_fluid_CG.CUF:398-509).  Offsets: the template and the D3Q27 ring
+ * of its surface nodes. */
+template <class S>
+__global__ __launch_bounds__(TPB) void k_shell(PartArgs a) {
+  const int p = blockIdx.x;
+  int c[3];
+  centre(a.pos_cur, p, c);
+  /* the particle's own nodes: MINDENS (every template node is non-fluid) */
+  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+    const size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
+    if ((a.key_cur[n] >> 3) != (uint32_t)p) continue;
+    a.rho_r[n] = 0.0f;
+    a.rho_b[n] = 0.0f;
+    a.phi[n] = (0.0f - 0.0f) / (0.0f + 0.0f);
+  }
+  /* the ring around its surface: densities of the fluid nodes */
+  for (int t = threadIdx.x; t < a.nring; t += TPB) {
+    const size_t n = node_at(a, c[0], c[1], c[2], a.ring[4 * t], a.ring[4 * t + 1], a.ring[4 * t + 2]);
+    const uint8_t f = a.fl_cur[n];
+    float rr, rb;
+    rho_sum<S>(a.pop_src, a.N, n, a.rho0, &rr, &rb);   /* issued with the type load */
+    if (f != FL_FLUID) continue;
+    a.rho_r[n] = rr;
+    a.rho_b[n] = rb;
+    a.phi[n] = (rr - rb) / (rr + rb);
+  }
+}
+
 /* ---------------- bounce-back on surface links ---------------------------
- * node_to_particle_bounce_back_bc2 (PART:1874-2113) and the moving-wall part
- * of particle_to_node_bounce_back_bc2_phase2 (PART:2201-2312), after the
- * collide-stream kernel: the post-collision value f_hi(ii) that the fluid
- * node ii pushed into the surface node s sits in dst(s, hi); the bounce-back
- * copy the fluid node made for itself sits in dst(ii, lo) and receives the
- * wall term here.  Every particle whose centre is within r^2 < rmax^2 of a
- * surface node acts on it, as in partBB_CG / partBB_CG_p2n2. */
+ * After the collide-stream kernel the post-collision value f_hi(ii) that the
+ * fluid node ii pushed into the surface node s sits in dst(s, hi).
+ *
+ * k_bb_links (particle_to_node_bounce_back_bc2_phase1 + phase2,
+ * kernels_fluid_PART.CUF:2153-2312): the bounced value goes back to the
+ * fluid node, dst(ii, lo) = dst(s, hi), minus the moving-wall term
+ * 6 w rho_k(ii) (e.v_b) of every particle whose centre lies within
+ * r^2 < rmax^2 of s (partBB_CG_p2n2 visits each of them), in ascending
+ * particle order.  Each surface node is handled once, by the particle that
+ * owns it, so every slot has one writer and no atomics are needed.
+ *
+ * k_bounce (node_to_particle_bounce_back_bc2, PART:1874-2113): momentum
+ * exchange on the same links for every particle whose ball holds s, summed
+ * in FP64 inside the workgroup. */
+
+/* integer minimum-image squared distance of node (gx,gy,gz) (0-based) from
+ * the centre c (1-based lattice coordinates) */
+__device__ __forceinline__ int ball_r2(const PartArgs& a, const int c[3], int gx, int gy, int gz) {
+  int d[3] = {gx + 1 - c[0], gy + 1 - c[1], gz + 1 - c[2]};
+  const int n[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (2 * d[k] > n[k]) d[k] -= n[k];
+    else if (2 * d[k] < -n[k]) d[k] += n[k];
+  }
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+/* what the link kernels read around one surface node s after the collide:
+ * the type and densities of its 18 D3Q19 neighbours and the 18 values the
+ * neighbours pushed into s (dst(s, l) comes from s - e_l, the value a fluid
+ * neighbour sent toward s in direction l) */
+template <class S>
+struct SurfaceNode {
+  size_t nb[19];
+  uint8_t f[19];
+  float rr[19], rb[19], pr[19], pb[19];
+  __device__ __forceinline__ void load(const PartArgs& a, size_t s, int gx, int gy, int gz) {
+#pragma unroll
+    for (int l = 1; l < 19; ++l) {
+      nb[l] = wrap_node(a, gx + EX(l), gy + EY(l), gz + EZ(l));
+      f[l] = a.fl_cur[nb[l]];
+      rr[l] = a.rho_r[nb[l]];
+      rb[l] = a.rho_b[nb[l]];
+      pr[l] = ldp<S>(a.pop_dst, (size_t)l * a.N + s, l, a.rho0);
+      pb[l] = ldp<S>(a.pop_dst, (size_t)(19 + l) * a.N + s, l, a.rho0);
+    }
+  }
+};
+
+template <class S>
+__global__ __launch_bounds__(TPB) void k_bb_links(PartArgs a) {
+  const int p = blockIdx.x;
+  /* the particles that may act on this particle's surface, with their
+     centre, velocity and lab-frame angular velocity */
+  __shared__ int sq[MAXNB], sc[MAXNB][3];
+  __shared__ double sv[MAXNB][3], so[MAXNB][3];
+  const int nnb = min(a.nnbr[p], MAXNB);
+  if ((int)threadIdx.x < nnb) {
+    const int k = threadIdx.x;
+    const int q = a.nbr[MAXNB * p + k];
+    sq[k] = q;
+    centre(a.pos_cur, q, sc[k]);
+    for (int d = 0; d < 3; ++d) sv[k][d] = a.pos_cur[13 * q + 3 + d];
+    double ot[3] = {0.0, 0.0, 0.0};
+    if (a.lrotate) omega_lab(a.pos_cur, q, ot);
+    for (int d = 0; d < 3; ++d) so[k][d] = ot[d];
+  }
+  __syncthreads();
+  int c[3];
+  centre(a.pos_cur, p, c);
+  const int r2max = a.rmax * a.rmax;
+  for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
+    int gx, gy, gz;
+    const size_t s = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
+    if (a.fl_cur[s] != FL_LIST || (a.key_cur[s] >> 3) != (uint32_t)p) continue;
+    /* one batch of loads: the 18 neighbours' types and densities and the
+       18 values pushed into s */
+    SurfaceNode<S> sn;
+    sn.load(a, s, gx, gy, gz);
+#pragma unroll
+    for (int il = 1; il <= 9; ++il) {
+      const int lo = 2 * il - 1, hi = 2 * il;
+      const bool fi = sn.f[lo] == FL_FLUID, fo = sn.f[hi] == FL_FLUID;
+      if (fi == fo) continue;
+      const int lin = fi ? lo : hi;      /* from s toward the fluid node   */
+      const int lout = fi ? hi : lo;     /* from the fluid node toward s   */
+      float fR = sn.pr[lout], fB = sn.pb[lout];
+      const float rr = sn.rr[lin], rb = sn.rb[lin];
+      for (int k = 0; k < nnb; ++k) {
+        if (ball_r2(a, sc[k], gx, gy, gz) >= r2max) continue;
+        double v[3] = {sv[k][0], sv[k][1], sv[k][2]};
+        if (a.lrotate) {
+          double rv[3], cr[3];
+          lever(a, a.pos_cur, sq[k], gx, gy, gz, true, rv);
+          const double rt[3] = {rv[0] + 0.5 * EX(lin), rv[1] + 0.5 * EY(lin), rv[2] + 0.5 * EZ(lin)};
+          const double ot[3] = {so[k][0], so[k][1], so[k][2]};
+          cross3(ot, rt, cr);
+          v[0] += cr[0]; v[1] += cr[1]; v[2] += cr[2];
+        }
+        const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
+        fR = fR - W(lout) * BUZ * rr * ev;
+        fB = fB - W(lout) * BUZ * rb * ev;
+      }
+      stp<S>(a.pop_dst, (size_t)lin * a.N + sn.nb[lin], lin, fR, a.rho0);
+      stp<S>(a.pop_dst, (size_t)(19 + lin) * a.N + sn.nb[lin], lin, fB, a.rho0);
+    }
+  }
+}
+
 template <class S>
 __global__ __launch_bounds__(TPB) void k_bounce(PartArgs a) {
   const int p = blockIdx.x;
@@ -465,18 +637,17 @@ __global__ __launch_bounds__(TPB) void k_bounce(PartArgs a) {
     int gx, gy, gz;
     const size_t s = node_at(a, c[0], c[1], c[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
     if (a.fl_cur[s] != FL_LIST) continue;
+    SurfaceNode<S> sn;
+    sn.load(a, s, gx, gy, gz);
     double rv[3] = {0, 0, 0};
     if (a.lrotate) lever(a, a.pos_cur, p, gx, gy, gz, true, rv);
 #pragma unroll
     for (int il = 1; il <= 9; ++il) {
       const int lo = 2 * il - 1, hi = 2 * il;
-      const size_t ii = wrap_node(a, gx + EX(lo), gy + EY(lo), gz + EZ(lo));
-      const size_t io = wrap_node(a, gx + EX(hi), gy + EY(hi), gz + EZ(hi));
-      const bool fi = a.fl_cur[ii] == FL_FLUID, fo = a.fl_cur[io] == FL_FLUID;
+      const bool fi = sn.f[lo] == FL_FLUID, fo = sn.f[hi] == FL_FLUID;
       if (fi == fo) continue;
       const int lin = fi ? lo : hi;      /* from s toward the fluid node   */
       const int lout = fi ? hi : lo;     /* from the fluid node toward s   */
-      const size_t nf = fi ? ii : io;
       double v[3] = {vs[0], vs[1], vs[2]};
       double rt[3] = {0, 0, 0};
       if (a.lrotate) {
@@ -488,11 +659,9 @@ __global__ __launch_bounds__(TPB) void k_bounce(PartArgs a) {
         v[0] += cr[0]; v[1] += cr[1]; v[2] += cr[2];
       }
       const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
-      const float wr = W(lout) * BUZ * a.rho_r[nf] * ev;
-      const float wb = W(lout) * BUZ * a.rho_b[nf] * ev;
-      const float fR = ldp<S>(a.pop_dst, (size_t)lout * a.N + s, lout, a.rho0);
-      const float fB = ldp<S>(a.pop_dst, (size_t)(19 + lout) * a.N + s, lout, a.rho0);
-      const double f2p = (double)((2.0f * fR - wr) + (2.0f * fB - wb));
+      const float wr = W(lout) * BUZ * sn.rr[lin] * ev;
+      const float wb = W(lout) * BUZ * sn.rb[lin] * ev;
+      const double f2p = (double)((2.0f * sn.pr[lout] - wr) + (2.0f * sn.pb[lout] - wb));
       double ft[3] = {f2p * EX(lout), f2p * EY(lout), f2p * EZ(lout)};
       acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
       if (a.lrotate) {
@@ -502,9 +671,6 @@ __global__ __launch_bounds__(TPB) void k_bounce(PartArgs a) {
         acc[4] += tq[1] * abs(EY(lout));
         acc[5] += tq[2] * abs(EZ(lout));
       }
-      /* moving wall: f_lin at the fluid node -= 6 w rho_k (e.v) */
-      atomic_sub<S>(a.pop_dst, (size_t)lin * a.N + nf, lin, wr, a.rho0);
-      atomic_sub<S>(a.pop_dst, (size_t)(19 + lin) * a.N + nf, lin, wb, a.rho0);
     }
   }
   block_reduce6(acc);
@@ -512,6 +678,55 @@ __global__ __launch_bounds__(TPB) void k_bounce(PartArgs a) {
     for (int d = 0; d < 3; ++d) a.myf[3 * p + d] = acc[d];
     if (a.lrotate)
       for (int d = 0; d < 3; ++d) a.myt[3 * p + d] = acc[3 + d];
+  }
+}
+
+/* neighbour list: particles j (i included) whose integer centres lie within
+ * 2 rmax of particle i, in ascending index order (wave ballots keep the
+ * order), so k_bb_links finds every particle whose ball reaches i's surface.
+ * More than MAXNB raises LBM_ERR_LIST_OVERFLOW, as makeListAtomsGPU raises
+ * on a full tile list (kernels_fluid_PART.CUF:2436-2439). */
+__global__ __launch_bounds__(TPB) void k_neigh(PartArgs a) {
+  const int i = blockIdx.x;
+  int ci[3];
+  centre(a.pos_cur, i, ci);
+  __shared__ int wcount[TPB / 64];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int lim = 4 * a.rmax * a.rmax;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < a.np; j0 += TPB) {
+    const int j = j0 + threadIdx.x;
+    bool hit = false;
+    if (j < a.np) {
+      int cj[3];
+      centre(a.pos_cur, j, cj);
+      int d[3] = {cj[0] - ci[0], cj[1] - ci[1], cj[2] - ci[2]};
+      const int n[3] = {a.nx, a.ny, a.nz};
+      for (int k = 0; k < 3; ++k) {
+        d[k] = ((d[k] % n[k]) + n[k]) % n[k];
+        if (2 * d[k] > n[k]) d[k] -= n[k];
+      }
+      hit = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < lim;
+    }
+    const unsigned long long m = __ballot(hit);
+    if (lane == 0) wcount[wid] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wid; ++w) off += wcount[w];
+    if (hit) {
+      const int pos = off + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < MAXNB) a.nbr[MAXNB * i + pos] = j;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 0; w < TPB / 64; ++w) base += wcount[w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.nnbr[i] = base;
+    if (base > MAXNB) raise_error(a.status, LBM_ERR_LIST_OVERFLOW, a.step);
   }
 }
 
@@ -523,11 +738,12 @@ struct PairParams {
 };
 
 __global__ __launch_bounds__(TPB) void k_pairs(PartArgs a, PairParams q) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  double f[3] = {0, 0, 0};
+  /* one workgroup per particle i, its threads over the partners j; FP64
+     partial forces reduced in a fixed order */
+  const int i = blockIdx.x;
+  double f[6] = {0, 0, 0, 0, 0, 0};
   const double xi = a.pos_cur[13 * i], yi = a.pos_cur[13 * i + 1], zi = a.pos_cur[13 * i + 2];
-  for (int j = 0; j < a.np; ++j) {
+  for (int j = threadIdx.x; j < a.np; j += TPB) {
     if (j == i) continue;
     /* addHertzian, PART:1385-1503, with the pair vector from i to j */
     double xdf = a.pos_cur[13 * j] - xi, ydf = a.pos_cur[13 * j + 1] - yi, zdf = a.pos_cur[13 * j + 2] - zi;
@@ -551,7 +767,9 @@ __global__ __launch_bounds__(TPB) void k_pairs(PartArgs a, PairParams q) {
       f[0] -= s * ux; f[1] -= s * uy; f[2] -= s * uz;
     }
   }
-  for (int d = 0; d < 3; ++d) a.myf2[3 * i + d] += f[d];
+  block_reduce6(f);
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) a.myf2[3 * i + d] += f[d];
 }
 
 /* ---------------- rigid-body update --------------------------------------
@@ -677,6 +895,10 @@ PartArgs make_pargs(lbm_ctx* c) {
   a.stamp_cur = c->d_stamp[c->cur];
   a.claim = c->d_claim;
   a.tmpl = c->d_tmpl; a.ntmpl = c->ntmpl;
+  a.ring = c->d_ring; a.nring = c->nring;
+  a.tlist = c->d_tlist; a.ntlist = c->ntlist;
+  a.nbr = c->d_nbr; a.nnbr = c->d_nnbr;
+  a.rmax = c->rmax;
   a.ball = c->d_ball; a.nball = c->nball;
   a.rho_r = c->rho_r; a.rho_b = c->rho_b; a.phi = c->phi;
   a.pop_src = c->pop[c->cur];
@@ -685,6 +907,8 @@ PartArgs make_pargs(lbm_ctx* c) {
   a.radius = c->cfg.radius;
   a.lrotate = c->cfg.lrotate;
   a.wetting = c->cfg.wetting;
+  a.rho_from_pops = c->variant == LBM_KERNEL_FUSED ? 1 : 0;
+  a.lag_from_pops = (c->variant == LBM_KERNEL_FUSED && c->step > 1) ? 1 : 0;
   a.status = c->d_status;
   a.step = (int)c->step;
   return a;
@@ -743,15 +967,26 @@ int lbm_particles_fill(lbm_ctx* c) {
 int lbm_particles_wetting(lbm_ctx* c) {
   if (c->cfg.wetting == LBM_WET_NONE) return 0;
   PartArgs a = make_pargs(c);
-  hipLaunchKernelGGL(k_wetting, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_wetting<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  return last_err();
+}
+
+int lbm_particles_shell(lbm_ctx* c) {
+  PartArgs a = make_pargs(c);
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shell<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
   return last_err();
 }
 
 int lbm_particles_bounce(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
-  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_bounce<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  hipLaunchKernelGGL(k_neigh, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  LBM_STORAGE_SWITCH(c->storage,
+    hipLaunchKernelGGL(k_bb_links<S>, dim3(c->np), dim3(TPB), 0, c->stream, a);
+    hipLaunchKernelGGL(k_bounce<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
   return last_err();
 }
+
+int lbm_particles_max_neighbours() { return MAXNB; }
 
 int lbm_particles_integrate(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
@@ -769,7 +1004,7 @@ int lbm_particles_integrate(lbm_ctx* c) {
   pp.hertz = g.hertz;
   pp.lub = g.lubrication;
   unsigned nb = (unsigned)((c->np + TPB - 1) / TPB);
-  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(TPB), 0, c->stream, a, pp);
+  hipLaunchKernelGGL(k_pairs, dim3(c->np), dim3(TPB), 0, c->stream, a, pp);
   IntParams ip;
   ip.rmass = g.rmass;
   ip.rotin = g.rmass > 0 ? 2.0 / 5.0 * (1.0 / (double)g.rmass) * R * R : 0.0;

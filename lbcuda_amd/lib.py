@@ -57,13 +57,21 @@ def lib():
     L.lbm_kernel_timing.argtypes = [vp, i32]
     L.lbm_get_kernel_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_int64)]
     L.lbm_kernel_info.argtypes = [vp, ctypes.c_char_p, i32, P(ctypes.c_double)]
+    L.lbm_set_kernel.argtypes = [vp, i32, i32, i32]
+    L.lbm_get_kernel.argtypes = [vp, P(i32), P(i32), P(i32)]
+    L.lbm_num_tiles.argtypes = []
+    L.lbm_num_tiles.restype = i32
+    L.lbm_tile_shape.argtypes = [i32, P(i32), P(i32), P(i32)]
+    L.lbm_autotune.argtypes = [vp, i32, P(i32), P(i32), P(ctypes.c_double)]
+    L.lbm_copy_bandwidth.argtypes = [ctypes.c_int64, i32, P(ctypes.c_double)]
     L.lbm_version.restype = ctypes.c_char_p
     L.lbm_fill_random_rho.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double, i32, i32, i32, vp, vp]
     L.lbm_fill_random_rho.restype = ctypes.c_int
     for name in ("lbm_create", "lbm_destroy", "lbm_init_fields", "lbm_set_pops", "lbm_set_particles",
                  "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments", "lbm_get_flags",
                  "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
-                 "lbm_get_kernel_timing", "lbm_kernel_info"):
+                 "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_set_kernel", "lbm_get_kernel",
+                 "lbm_tile_shape", "lbm_autotune", "lbm_copy_bandwidth"):
         getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -72,7 +80,9 @@ def lib():
 EXPORTED = ["lbm_config_default", "lbm_create", "lbm_destroy", "lbm_init_fields", "lbm_set_pops",
             "lbm_set_particles", "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments",
             "lbm_get_flags", "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
-            "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version", "lbm_fill_random_rho"]
+            "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version", "lbm_fill_random_rho",
+            "lbm_set_kernel", "lbm_get_kernel", "lbm_num_tiles", "lbm_tile_shape", "lbm_autotune",
+            "lbm_copy_bandwidth"]
 
 
 def _f32(a):
@@ -187,6 +197,25 @@ class Box:
         self._check(lib().lbm_get_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)), "lbm_get_kernel_timing")
         return ms.value, n.value
 
+    def set_kernel(self, variant, tile=0, zchunk=0):
+        """variant KERNEL_FUSED (tile index, z chunk; 0 = default chunk) or
+        KERNEL_TWOPASS."""
+        self._check(lib().lbm_set_kernel(self.h, variant, tile, zchunk), "lbm_set_kernel")
+
+    def get_kernel(self):
+        v, t, z = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(lib().lbm_get_kernel(self.h, ctypes.byref(v), ctypes.byref(t), ctypes.byref(z)),
+                    "lbm_get_kernel")
+        return v.value, t.value, z.value
+
+    def autotune(self, reps=5):
+        """Time every valid (tile, z chunk) of the fused kernel and keep the
+        fastest; returns (tile, zchunk, ms per launch)."""
+        t, z, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        self._check(lib().lbm_autotune(self.h, reps, ctypes.byref(t), ctypes.byref(z), ctypes.byref(ms)),
+                    "lbm_autotune")
+        return t.value, z.value, ms.value
+
     def kernel_info(self):
         buf = ctypes.create_string_buffer(128)
         b = ctypes.c_double()
@@ -203,6 +232,28 @@ def fill_random_rho(shape, seed, mean=0.5, amp=0.05):
     if rc:
         raise LbmError(rc, "lbm_fill_random_rho")
     return rr, rb
+
+
+def tile_shapes():
+    """The fused kernel's tile table: list of (tx, ty, wrap)."""
+    L = lib()
+    out = []
+    for t in range(L.lbm_num_tiles()):
+        a, b, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        if L.lbm_tile_shape(t, ctypes.byref(a), ctypes.byref(b), ctypes.byref(w)):
+            raise LbmError(1, "lbm_tile_shape")
+        out.append((a.value, b.value, w.value))
+    return out
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """Measured device copy bandwidth (GB/s, read + write) of a plain copy
+    kernel over two buffers of nbytes."""
+    g = ctypes.c_double()
+    rc = lib().lbm_copy_bandwidth(nbytes, reps, ctypes.byref(g))
+    if rc:
+        raise LbmError(rc, "lbm_copy_bandwidth")
+    return g.value
 
 
 def version():

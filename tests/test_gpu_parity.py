@@ -7,7 +7,7 @@ Tolerances (FP32 arithmetic on the GPU vs the FP64 oracle):
   particle positions       |d| <= 1e-6 lattice units;
   particle forces          |d| <= 1e-4 * (1 + |F|).
 Shifted 16-bit storage: relative density error vs the FP64 oracle
-<= 1e-3 (fp16) and <= 1e-2 (bf16).
+<= 1e-3 (fp16) and <= 2e-2 (bf16, 8-bit mantissa) after 100 steps.
 Populations on non-fluid nodes are not compared: there the reference keeps
 stale values that never reach the fluid (no collision, :689).
 """
@@ -15,7 +15,8 @@ import numpy as np
 import pytest
 
 from lbcuda_amd.abi import (default_config, INIT_RHO, INIT_VELFIELD, INIT_DROPLET, INIT_ZPLANES,
-                            INIT_CYLINDER, WET_NEUTRAL, WET_JANUS, STORE_F16, STORE_BF16)
+                            INIT_CYLINDER, WET_NEUTRAL, WET_JANUS, STORE_F16, STORE_BF16, KERNEL_FUSED,
+                            KERNEL_TWOPASS)
 from lbcuda_amd import workloads as wl
 from lbcuda_amd.lib import Box
 from oracle.pyoracle import OracleBox
@@ -23,8 +24,10 @@ from oracle.pyoracle import OracleBox
 pytestmark = pytest.mark.gpu
 
 
-def both(cfg, init, pos=None, **kw):
+def both(cfg, init, pos=None, variant=KERNEL_FUSED, **kw):
     g = Box(cfg)
+    if variant != KERNEL_FUSED:
+        g.set_kernel(variant)
     o = OracleBox(cfg, 64)
     if pos is not None:
         g.set_particles(pos)
@@ -41,13 +44,14 @@ def maxdiff(a, b, mask=None):
     return float(d.max()) if d.size else 0.0
 
 
+@pytest.mark.parametrize("variant", [KERNEL_FUSED, KERNEL_TWOPASS])
 @pytest.mark.parametrize("shape", [(24, 24, 24), (40, 20, 12), (64, 64, 16)])
-def test_spinodal_parity(shape):
-    print(f"spinodal parity shape={shape}", flush=True)
+def test_spinodal_parity(shape, variant):
+    print(f"spinodal parity shape={shape} variant={variant}", flush=True)
     cfg = default_config(shape[0], sigma=0.02, beta=0.7)
     cfg.nx, cfg.ny, cfg.nz = shape
     rr, rb = wl.spinodal_rho(shape, seed=101)
-    g, o = both(cfg, INIT_RHO, rho_r=rr, rho_b=rb)
+    g, o = both(cfg, INIT_RHO, rho_r=rr, rho_b=rb, variant=variant)
     assert g.step(30) == 0 and o.step(30) == 0
     for c in (0, 1):
         assert maxdiff(g.pops(c), o.pops(c)) < 2e-5
@@ -81,7 +85,8 @@ def test_taylor_green_parity():
     f = o.pops(0)
     ex = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0])
     u_o = (f * ex).sum(-1) / f.sum(-1)
-    assert maxdiff(velg[0], u_o) < 1e-7
+    # FP32 moments of populations ~0.05 after 100 FP32 steps: a few ulp of 1
+    assert maxdiff(velg[0], u_o) < 5e-7
 
 
 def test_global_sums_fp64():
@@ -124,11 +129,12 @@ def particle_case(n=32, npart=3, seed=11, lrotate=1, rmass=1.0 / 1500.0, vel=0.0
     return cfg, pos, rr, rb
 
 
+@pytest.mark.parametrize("variant", [KERNEL_FUSED, KERNEL_TWOPASS])
 @pytest.mark.parametrize("wetting,lrotate", [(WET_NEUTRAL, 1), (WET_JANUS, 1), (WET_NEUTRAL, 0)])
-def test_particles_parity(wetting, lrotate):
-    print(f"particles parity wetting={wetting} lrotate={lrotate}", flush=True)
+def test_particles_parity(wetting, lrotate, variant):
+    print(f"particles parity wetting={wetting} lrotate={lrotate} variant={variant}", flush=True)
     cfg, pos, rr, rb = particle_case(npart=3, lrotate=lrotate, wetting=wetting)
-    g, o = both(cfg, INIT_RHO, pos=pos, rho_r=rr, rho_b=rb)
+    g, o = both(cfg, INIT_RHO, pos=pos, rho_r=rr, rho_b=rb, variant=variant)
     assert (g.flags() == o.flags()).all()
     for step in range(4):
         assert g.step(10) == 0 and o.step(10) == 0
@@ -149,10 +155,11 @@ def test_particles_parity(wetting, lrotate):
     assert np.abs(p_end[0:3] - pos[0:3]).max() > 0.5   # the particles crossed nodes
 
 
-def test_prescribed_velocity_sphere_parity():
+@pytest.mark.parametrize("variant", [KERNEL_FUSED, KERNEL_TWOPASS])
+def test_prescribed_velocity_sphere_parity(variant):
     """A sphere with 1/m = 0 keeps its velocity (config: translating sphere):
     fill/erase every few steps, drag history vs the oracle."""
-    print("prescribed sphere", flush=True)
+    print(f"prescribed sphere variant={variant}", flush=True)
     n = 32
     cfg = default_config(n, sigma=0.0, beta=0.0, with_particles=1, n_particles=1, rmass=0.0, max_vel=1.0,
                          wetting=WET_NEUTRAL)
@@ -162,7 +169,7 @@ def test_prescribed_velocity_sphere_parity():
     pos[9, 0] = 1.0
     rr = np.ones((n, n, n), np.float32)
     rb = np.zeros_like(rr)
-    g, o = both(cfg, INIT_RHO, pos=pos, rho_r=rr, rho_b=rb)
+    g, o = both(cfg, INIT_RHO, pos=pos, rho_r=rr, rho_b=rb, variant=variant)
     for _ in range(6):
         assert g.step(10) == 0 and o.step(10) == 0
         assert (g.flags() == o.flags()).all()
@@ -171,7 +178,7 @@ def test_prescribed_velocity_sphere_parity():
         assert np.all(np.abs(gf - of) <= 1e-4 * (1 + np.abs(of)))
 
 
-@pytest.mark.parametrize("storage,tol", [(STORE_F16, 1e-3), (STORE_BF16, 1e-2)])
+@pytest.mark.parametrize("storage,tol", [(STORE_F16, 1e-3), (STORE_BF16, 2e-2)])
 def test_low_precision_storage_error(storage, tol):
     print(f"storage {storage}", flush=True)
     n = 32
