@@ -33,15 +33,20 @@ def log(msg):
 
 
 def traffic_from_profiles(storage_tag):
-    """Measured HBM bytes per collide-stream launch, from the committed rocprof
-    counter summary (profiles/collide_push_256_traffic.md), else None."""
-    path = os.path.join(ROOT, "profiles", "collide_push_256_traffic.md")
+    """HBM bytes per launch of the fused collide-stream kernel, from the
+    committed rocprofv3 counter summary (profiles/k_fused_256_pickering.md:
+    FETCH_SIZE and WRITE_SIZE scaled by the same counters of a copy kernel of
+    known size), else None."""
+    path = os.path.join(ROOT, "profiles", "k_fused_256_pickering.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        m = re.match(rf"\|\s*{storage_tag}\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$", line.strip())
-        if m:
-            return float(m.group(1))
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == storage_tag and cells[1].startswith("k_fused"):
+            try:
+                return float(cells[5])
+            except ValueError:
+                return None
     return None
 
 
@@ -113,7 +118,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--particles", type=int, default=1280)
     ap.add_argument("--prof-steps", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -135,6 +140,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(args, rr, rb, pos)
 
     traffic = traffic_from_profiles("f32")
+    traffic16 = traffic_from_profiles("f16")
     out = {
         "metric": "MLUPS @256^3 CG+1280 colloids on 1 GPU; % HBM peak; FP16-store vs FP32 speedup",
         "value": round(r32["mlups"], 1),
@@ -165,7 +171,8 @@ def main():
                              "speedup_vs_f32": round(r16["mlups"] / r32["mlups"], 3),
                              "kernel_achieved_gbs": round(r16["achieved_gbs"], 1),
                              "kernel_frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                             "kernel_ms": round(r16["kernel_ms"], 4)}
+                             "kernel_ms": round(r16["kernel_ms"], 4), "kernel": r16["kernel"],
+                             "traffic": traffic16}
     print(json.dumps(out), flush=True)
 
 

@@ -14,6 +14,7 @@ module lbcuda_amd
   integer(c_int32_t), parameter :: LBM_INIT_DROPLET = 1, LBM_INIT_ZPLANES = 2, LBM_INIT_RHO = 3, &
                                    LBM_INIT_POPS = 4, LBM_INIT_CYLINDER = 5, LBM_INIT_RANDOM = 6, &
                                    LBM_INIT_VELFIELD = 7
+  integer(c_int32_t), parameter :: LBM_KERNEL_FUSED = 0, LBM_KERNEL_TWOPASS = 1
 
   ! struct lbm_config, field for field
   type, bind(C) :: lbm_config
@@ -110,6 +111,20 @@ module lbcuda_amd
       real(c_double), value :: mean, amp
       integer(c_int32_t), value :: nx, ny, nz
       type(c_ptr), value :: rho_r, rho_b
+    end function
+
+    integer(c_int) function lbm_set_kernel(ctx, variant, tile, zchunk) bind(C, name="lbm_set_kernel")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: variant, tile, zchunk
+    end function
+
+    integer(c_int) function lbm_autotune(ctx, reps, tile, zchunk, best_ms) bind(C, name="lbm_autotune")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: reps
+      integer(c_int32_t), intent(out) :: tile, zchunk
+      real(c_double), intent(out) :: best_ms
     end function
   end interface
 end module lbcuda_amd

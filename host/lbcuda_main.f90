@@ -8,7 +8,7 @@
 ! optional namelist file lbcuda.nml:
 !   &lbcuda box=256, taur=1.0, taub=1.0, sigma=0.1, beta=0.999, storage='f32',
 !           wetting='janus', radius=5.5, rmass=6.6667e-4, lubrication=.false.,
-!           seed=1234 /
+!           seed=1234, autotune=.false. /
 ! The time loop replaces the chevron launches of main.CUF:125-308 with
 ! lbm_step calls of nIterOut steps and prints the reference's progress line
 ! (main.CUF:282-289; Mlup/s counts (n+2)^3 nodes there) plus the
@@ -37,12 +37,15 @@ program lbcuda_host
   integer(8) :: c0, c1, crate
   real(8) :: secs
   integer :: done, chunk
-  namelist /lbcuda/ box, taur, taub, sigma, beta, storage, wetting, radius, rmass, lubrication, seed
+  logical :: autotune
+  integer(c_int32_t) :: tile, zchunk
+  real(c_double) :: best_ms
+  namelist /lbcuda/ box, taur, taub, sigma, beta, storage, wetting, radius, rmass, lubrication, seed, autotune
 
   ! defaults: dimensions_m.CUF:7 (MYDIMESION 64), :50, :59-60, :103-104
   box = 64; taur = 1.0; taub = 1.0; sigma = 0.1; beta = 0.999
   storage = 'f32'; wetting = 'janus'; radius = 5.5; rmass = 1.0/1500.0
-  lubrication = .false.; seed = 1234
+  lubrication = .false.; seed = 1234; autotune = .false.
 
   inquire(file='lbcuda.nml', exist=lexist)
   if (lexist) then
@@ -171,6 +174,14 @@ program lbcuda_host
     stop 1
   end select
   call check(rc, 'lbm_init_fields')
+
+  ! tile of the collide-stream kernel: the reference fixes TILE1..3 at compile
+  ! time (defines.h:14-24); here the library times its tile table on this box
+  if (autotune) then
+    rc = lbm_autotune(ctx, 3_c_int32_t, tile, zchunk, best_ms)
+    call check(rc, 'lbm_autotune')
+    write(*,'(a,i3,a,i4,a,f9.3,a)') ' autotune: tile', tile, ' z-chunk', zchunk, ' (', best_ms, ' ms)'
+  end if
 
   rc = lbm_get_sums(ctx, sums)
   write(*,'(a,2es16.8)') ' initial mass R,B:', sums(1), sums(2)

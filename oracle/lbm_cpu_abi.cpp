@@ -126,4 +126,18 @@ int lbm_fill_random_rho(uint64_t seed, double mean, double amp, int32_t nx, int3
   return 0;
 }
 
+/* The CPU implementation has a single collide-stream path and no tiles:
+ * the kernel-selection calls accept the request and report tile 0. */
+int lbm_set_kernel(lbm_ctx* c, int32_t variant, int32_t, int32_t) {
+  return (c && (variant == LBM_KERNEL_FUSED || variant == LBM_KERNEL_TWOPASS)) ? 0 : LBM_ERR_ARG;
+}
+
+int lbm_autotune(lbm_ctx* c, int32_t reps, int32_t* tile, int32_t* zchunk, double* best_ms) {
+  if (!c || reps <= 0) return LBM_ERR_ARG;
+  if (tile) *tile = 0;
+  if (zchunk) *zchunk = 1;
+  if (best_ms) *best_ms = 0.0;
+  return 0;
+}
+
 }  /* extern "C" */
