@@ -65,6 +65,8 @@ struct PartArgs {
   int* nbr;                /* (MAXNB, np) particles whose ball can reach
                               this particle's template, ascending       */
   int* nnbr;               /* (np) number found                         */
+  double* contrib;         /* (6, MAXNB, np) link momentum a particle's
+                              surface gives to each of its neighbours  */
   int rmax;
   const int8_t* ball;      /* (4, nball), [3] = 1 if r^2 < rmax^2 */
   int nball;
@@ -514,19 +516,21 @@ __global__ __launch_bounds__(TPB) void k_shell(PartArgs a) {
 
 /* ---------------- bounce-back on surface links ---------------------------
  * After the collide-stream kernel the post-collision value f_hi(ii) that the
- * fluid node ii pushed into the surface node s sits in dst(s, hi).
+ * fluid node ii pushed into the surface node s sits in dst(s, hi).  One pass
+ * over the surface nodes, each handled by the particle that owns it:
  *
- * k_bb_links (particle_to_node_bounce_back_bc2_phase1 + phase2,
- * kernels_fluid_PART.CUF:2153-2312): the bounced value goes back to the
- * fluid node, dst(ii, lo) = dst(s, hi), minus the moving-wall term
- * 6 w rho_k(ii) (e.v_b) of every particle whose centre lies within
- * r^2 < rmax^2 of s (partBB_CG_p2n2 visits each of them), in ascending
- * particle order.  Each surface node is handled once, by the particle that
- * owns it, so every slot has one writer and no atomics are needed.
- *
- * k_bounce (node_to_particle_bounce_back_bc2, PART:1874-2113): momentum
- * exchange on the same links for every particle whose ball holds s, summed
- * in FP64 inside the workgroup. */
+ *  - bounce-back (particle_to_node_bounce_back_bc2_phase1 + phase2,
+ *    kernels_fluid_PART.CUF:2153-2312): dst(ii, lo) = dst(s, hi) minus the
+ *    moving-wall term 6 w rho_k(ii) (e.v_b) of every particle whose centre
+ *    lies within r^2 < rmax^2 of s (partBB_CG_p2n2 visits each of them), in
+ *    ascending particle order; every slot has one writer, no atomics;
+ *  - momentum exchange (node_to_particle_bounce_back_bc2, PART:1874-2113) on
+ *    the same links for every such particle q: the owner's own share is
+ *    summed in FP64 over the workgroup into myf/myt; the share of another
+ *    particle q (two particles in contact) is summed the same way into
+ *    contrib(slot of q) and added to q's force, in ascending owner order, by
+ *    k_integrate.  The sums never depend on scheduling.
+ */
 
 /* integer minimum-image squared distance of node (gx,gy,gz) (0-based) from
  * the centre c (1-based lattice coordinates) */
@@ -541,10 +545,9 @@ __device__ __forceinline__ int ball_r2(const PartArgs& a, const int c[3], int gx
   return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
 }
 
-/* what the link kernels read around one surface node s after the collide:
+/* what the link pass reads around one surface node s after the collide:
  * the type and densities of its 18 D3Q19 neighbours and the 18 values the
- * neighbours pushed into s (dst(s, l) comes from s - e_l, the value a fluid
- * neighbour sent toward s in direction l) */
+ * neighbours pushed into s (dst(s, l) comes from s - e_l), in one batch */
 template <class S>
 struct SurfaceNode {
   size_t nb[19];
@@ -563,127 +566,149 @@ struct SurfaceNode {
   }
 };
 
+/* neighbour-list entry cached in LDS */
+struct NbrCache {
+  int q[MAXNB];
+  int c[MAXNB][3];
+  double v[MAXNB][3];
+  double o[MAXNB][3];
+};
+
+/* wall velocity e_lout . v_b of particle k (cache slot) at the link of s */
+__device__ __forceinline__ void link_velocity(const PartArgs& a, const NbrCache& nc, int k, int gx, int gy, int gz,
+                                              int lin, double v[3], double rt[3]) {
+  v[0] = nc.v[k][0]; v[1] = nc.v[k][1]; v[2] = nc.v[k][2];
+  rt[0] = rt[1] = rt[2] = 0.0;
+  if (a.lrotate) {
+    double rv[3], cr[3];
+    lever(a, a.pos_cur, nc.q[k], gx, gy, gz, true, rv);
+    rt[0] = rv[0] + 0.5 * EX(lin); rt[1] = rv[1] + 0.5 * EY(lin); rt[2] = rv[2] + 0.5 * EZ(lin);
+    const double ot[3] = {nc.o[k][0], nc.o[k][1], nc.o[k][2]};
+    cross3(ot, rt, cr);
+    v[0] += cr[0]; v[1] += cr[1]; v[2] += cr[2];
+  }
+}
+
+/* momentum given to particle k by the links of surface node s
+ * (addForce at :1964-1972 / :2049-2057, torque :1999-2001) */
 template <class S>
-__global__ __launch_bounds__(TPB) void k_bb_links(PartArgs a) {
+__device__ __forceinline__ void link_force(const PartArgs& a, const NbrCache& nc, int k, const SurfaceNode<S>& sn,
+                                           int gx, int gy, int gz, double acc[6]) {
+#pragma unroll
+  for (int il = 1; il <= 9; ++il) {
+    const int lo = 2 * il - 1, hi = 2 * il;
+    const bool fi = sn.f[lo] == FL_FLUID, fo = sn.f[hi] == FL_FLUID;
+    if (fi == fo) continue;
+    const int lin = fi ? lo : hi;      /* from s toward the fluid node   */
+    const int lout = fi ? hi : lo;     /* from the fluid node toward s   */
+    double v[3], rt[3];
+    link_velocity(a, nc, k, gx, gy, gz, lin, v, rt);
+    const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
+    const float wr = W(lout) * BUZ * sn.rr[lin] * ev;
+    const float wb = W(lout) * BUZ * sn.rb[lin] * ev;
+    const double f2p = (double)((2.0f * sn.pr[lout] - wr) + (2.0f * sn.pb[lout] - wb));
+    const double ft[3] = {f2p * EX(lout), f2p * EY(lout), f2p * EZ(lout)};
+    acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
+    if (a.lrotate) {
+      double tq[3];
+      cross3(rt, ft, tq);
+      acc[3] += tq[0] * abs(EX(lout));
+      acc[4] += tq[1] * abs(EY(lout));
+      acc[5] += tq[2] * abs(EZ(lout));
+    }
+  }
+}
+
+template <class S>
+__global__ __launch_bounds__(TPB) void k_links(PartArgs a) {
   const int p = blockIdx.x;
-  /* the particles that may act on this particle's surface, with their
-     centre, velocity and lab-frame angular velocity */
-  __shared__ int sq[MAXNB], sc[MAXNB][3];
-  __shared__ double sv[MAXNB][3], so[MAXNB][3];
+  __shared__ NbrCache nc;
+  __shared__ int self_slot;
   const int nnb = min(a.nnbr[p], MAXNB);
+  if (threadIdx.x == 0) self_slot = -1;
+  __syncthreads();
   if ((int)threadIdx.x < nnb) {
     const int k = threadIdx.x;
     const int q = a.nbr[MAXNB * p + k];
-    sq[k] = q;
-    centre(a.pos_cur, q, sc[k]);
-    for (int d = 0; d < 3; ++d) sv[k][d] = a.pos_cur[13 * q + 3 + d];
+    nc.q[k] = q;
+    centre(a.pos_cur, q, nc.c[k]);
+    for (int d = 0; d < 3; ++d) nc.v[k][d] = a.pos_cur[13 * q + 3 + d];
     double ot[3] = {0.0, 0.0, 0.0};
     if (a.lrotate) omega_lab(a.pos_cur, q, ot);
-    for (int d = 0; d < 3; ++d) so[k][d] = ot[d];
+    for (int d = 0; d < 3; ++d) nc.o[k][d] = ot[d];
+    if (q == p) self_slot = k;
   }
   __syncthreads();
   int c[3];
   centre(a.pos_cur, p, c);
   const int r2max = a.rmax * a.rmax;
+  const int kp = self_slot;
+  /* a full neighbour list (LBM_ERR_LIST_OVERFLOW raised by k_neigh) may miss
+     the particle itself: leave its link sums untouched then */
+  if (kp < 0) return;
+  double own[6] = {0, 0, 0, 0, 0, 0};
+  bool shared_node = false;   /* some surface node of this thread is covered by another particle */
   for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
     int gx, gy, gz;
     const size_t s = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
     if (a.fl_cur[s] != FL_LIST || (a.key_cur[s] >> 3) != (uint32_t)p) continue;
-    /* one batch of loads: the 18 neighbours' types and densities and the
-       18 values pushed into s */
     SurfaceNode<S> sn;
     sn.load(a, s, gx, gy, gz);
+    unsigned cover = 0;   /* cache slots of the particles whose ball holds s */
+    for (int k = 0; k < nnb; ++k)
+      if (ball_r2(a, nc.c[k], gx, gy, gz) < r2max) cover |= 1u << k;
+    if (cover & ~(1u << kp)) shared_node = true;
 #pragma unroll
     for (int il = 1; il <= 9; ++il) {
       const int lo = 2 * il - 1, hi = 2 * il;
       const bool fi = sn.f[lo] == FL_FLUID, fo = sn.f[hi] == FL_FLUID;
       if (fi == fo) continue;
-      const int lin = fi ? lo : hi;      /* from s toward the fluid node   */
-      const int lout = fi ? hi : lo;     /* from the fluid node toward s   */
+      const int lin = fi ? lo : hi;
+      const int lout = fi ? hi : lo;
       float fR = sn.pr[lout], fB = sn.pb[lout];
-      const float rr = sn.rr[lin], rb = sn.rb[lin];
       for (int k = 0; k < nnb; ++k) {
-        if (ball_r2(a, sc[k], gx, gy, gz) >= r2max) continue;
-        double v[3] = {sv[k][0], sv[k][1], sv[k][2]};
-        if (a.lrotate) {
-          double rv[3], cr[3];
-          lever(a, a.pos_cur, sq[k], gx, gy, gz, true, rv);
-          const double rt[3] = {rv[0] + 0.5 * EX(lin), rv[1] + 0.5 * EY(lin), rv[2] + 0.5 * EZ(lin)};
-          const double ot[3] = {so[k][0], so[k][1], so[k][2]};
-          cross3(ot, rt, cr);
-          v[0] += cr[0]; v[1] += cr[1]; v[2] += cr[2];
-        }
+        if (!(cover >> k & 1u)) continue;
+        double v[3], rt[3];
+        link_velocity(a, nc, k, gx, gy, gz, lin, v, rt);
         const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
-        fR = fR - W(lout) * BUZ * rr * ev;
-        fB = fB - W(lout) * BUZ * rb * ev;
+        fR = fR - W(lout) * BUZ * sn.rr[lin] * ev;
+        fB = fB - W(lout) * BUZ * sn.rb[lin] * ev;
       }
       stp<S>(a.pop_dst, (size_t)lin * a.N + sn.nb[lin], lin, fR, a.rho0);
       stp<S>(a.pop_dst, (size_t)(19 + lin) * a.N + sn.nb[lin], lin, fB, a.rho0);
     }
+    link_force<S>(a, nc, kp, sn, gx, gy, gz, own);
   }
-}
-
-template <class S>
-__global__ __launch_bounds__(TPB) void k_bounce(PartArgs a) {
-  const int p = blockIdx.x;
-  int c[3];
-  centre(a.pos_cur, p, c);
-  const double vs[3] = {a.pos_cur[13 * p + 3], a.pos_cur[13 * p + 4], a.pos_cur[13 * p + 5]};
-  double ot[3] = {0, 0, 0};
-  if (a.lrotate) omega_lab(a.pos_cur, p, ot);
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = threadIdx.x; t < a.nball; t += TPB) {
-    if (!a.ball[4 * t + 3]) continue;   /* r^2 < sphereMax */
-    int gx, gy, gz;
-    const size_t s = node_at(a, c[0], c[1], c[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
-    if (a.fl_cur[s] != FL_LIST) continue;
-    SurfaceNode<S> sn;
-    sn.load(a, s, gx, gy, gz);
-    double rv[3] = {0, 0, 0};
-    if (a.lrotate) lever(a, a.pos_cur, p, gx, gy, gz, true, rv);
-#pragma unroll
-    for (int il = 1; il <= 9; ++il) {
-      const int lo = 2 * il - 1, hi = 2 * il;
-      const bool fi = sn.f[lo] == FL_FLUID, fo = sn.f[hi] == FL_FLUID;
-      if (fi == fo) continue;
-      const int lin = fi ? lo : hi;      /* from s toward the fluid node   */
-      const int lout = fi ? hi : lo;     /* from the fluid node toward s   */
-      double v[3] = {vs[0], vs[1], vs[2]};
-      double rt[3] = {0, 0, 0};
-      if (a.lrotate) {
-        rt[0] = rv[0] + 0.5 * EX(lin);
-        rt[1] = rv[1] + 0.5 * EY(lin);
-        rt[2] = rv[2] + 0.5 * EZ(lin);
-        double cr[3];
-        cross3(ot, rt, cr);
-        v[0] += cr[0]; v[1] += cr[1]; v[2] += cr[2];
-      }
-      const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
-      const float wr = W(lout) * BUZ * sn.rr[lin] * ev;
-      const float wb = W(lout) * BUZ * sn.rb[lin] * ev;
-      const double f2p = (double)((2.0f * sn.pr[lout] - wr) + (2.0f * sn.pb[lout] - wb));
-      double ft[3] = {f2p * EX(lout), f2p * EY(lout), f2p * EZ(lout)};
-      acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
-      if (a.lrotate) {
-        double tq[3];
-        cross3(rt, ft, tq);
-        acc[3] += tq[0] * abs(EX(lout));
-        acc[4] += tq[1] * abs(EY(lout));
-        acc[5] += tq[2] * abs(EZ(lout));
-      }
-    }
-  }
-  block_reduce6(acc);
+  block_reduce6(own);
   if (threadIdx.x == 0) {
-    for (int d = 0; d < 3; ++d) a.myf[3 * p + d] = acc[d];
+    for (int d = 0; d < 3; ++d) a.myf[3 * p + d] = own[d];
     if (a.lrotate)
-      for (int d = 0; d < 3; ++d) a.myt[3 * p + d] = acc[3 + d];
+      for (int d = 0; d < 3; ++d) a.myt[3 * p + d] = own[3 + d];
+  }
+  /* shares of the other particles in contact with this one */
+  const bool any_shared = __syncthreads_or(shared_node);
+  for (int k = 0; k < nnb; ++k) {
+    if (k == kp) continue;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (any_shared)
+      for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
+        int gx, gy, gz;
+        const size_t s = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
+        if (a.fl_cur[s] != FL_LIST || (a.key_cur[s] >> 3) != (uint32_t)p) continue;
+        if (ball_r2(a, nc.c[k], gx, gy, gz) >= r2max) continue;
+        SurfaceNode<S> sn;
+        sn.load(a, s, gx, gy, gz);
+        link_force<S>(a, nc, k, sn, gx, gy, gz, acc);
+      }
+    block_reduce6(acc);
+    if (threadIdx.x == 0)
+      for (int d = 0; d < 6; ++d) a.contrib[6 * ((size_t)MAXNB * p + k) + d] = acc[d];
   }
 }
 
 /* neighbour list: particles j (i included) whose integer centres lie within
  * 2 rmax of particle i, in ascending index order (wave ballots keep the
- * order), so k_bb_links finds every particle whose ball reaches i's surface.
+ * order), so k_links finds every particle whose ball reaches i's surface.
  * More than MAXNB raises LBM_ERR_LIST_OVERFLOW, as makeListAtomsGPU raises
  * on a full tile list (kernels_fluid_PART.CUF:2436-2439). */
 __global__ __launch_bounds__(TPB) void k_neigh(PartArgs a) {
@@ -788,14 +813,28 @@ __global__ __launch_bounds__(TPB) void k_integrate(PartArgs a, IntParams q) {
   double* Pn = a.pos_new + 13 * i;
   double* Fc = a.f_cur + 12 * i;
   double* Fo = a.f_old + 12 * i;
+  /* link momentum: own surface, then the surfaces of particles in contact
+     (their owners' shares for i, ascending owner index) */
+  double fb[6] = {a.myf[3 * i], a.myf[3 * i + 1], a.myf[3 * i + 2], 0.0, 0.0, 0.0};
+  if (a.lrotate)
+    for (int d = 0; d < 3; ++d) fb[3 + d] = a.myt[3 * i + d];
+  const int nni = min(a.nnbr[i], MAXNB);
+  for (int k = 0; k < nni; ++k) {
+    const int p = a.nbr[MAXNB * i + k];
+    if (p == i) continue;
+    const int nnp = min(a.nnbr[p], MAXNB);
+    for (int j = 0; j < nnp; ++j)
+      if (a.nbr[MAXNB * p + j] == i)
+        for (int d = 0; d < 6; ++d) fb[d] += a.contrib[6 * ((size_t)MAXNB * p + j) + d];
+  }
   /* addForce_particle_bounce_back: F(t) = F_other + (F_bb(t+1/2)+F_bb(t-1/2))/2 */
   for (int d = 0; d < 3; ++d) {
-    Fc[3 + d] = a.myf[3 * i + d];
+    Fc[3 + d] = fb[d];
     Fc[d] = a.myf2[3 * i + d] + (Fc[3 + d] + Fo[3 + d]) * 0.5;
   }
   if (a.lrotate)
     for (int d = 0; d < 3; ++d) {
-      Fc[9 + d] = a.myt[3 * i + d];
+      Fc[9 + d] = fb[3 + d];
       Fc[6 + d] = a.myt2[3 * i + d] + (Fc[9 + d] + Fo[9 + d]) * 0.5;
     }
   /* nve_lf */
@@ -897,7 +936,7 @@ PartArgs make_pargs(lbm_ctx* c) {
   a.tmpl = c->d_tmpl; a.ntmpl = c->ntmpl;
   a.ring = c->d_ring; a.nring = c->nring;
   a.tlist = c->d_tlist; a.ntlist = c->ntlist;
-  a.nbr = c->d_nbr; a.nnbr = c->d_nnbr;
+  a.nbr = c->d_nbr; a.nnbr = c->d_nnbr; a.contrib = c->d_contrib;
   a.rmax = c->rmax;
   a.ball = c->d_ball; a.nball = c->nball;
   a.rho_r = c->rho_r; a.rho_b = c->rho_b; a.phi = c->phi;
@@ -981,8 +1020,7 @@ int lbm_particles_bounce(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
   hipLaunchKernelGGL(k_neigh, dim3(c->np), dim3(TPB), 0, c->stream, a);
   LBM_STORAGE_SWITCH(c->storage,
-    hipLaunchKernelGGL(k_bb_links<S>, dim3(c->np), dim3(TPB), 0, c->stream, a);
-    hipLaunchKernelGGL(k_bounce<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+    hipLaunchKernelGGL(k_links<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
   return last_err();
 }
 

@@ -156,6 +156,40 @@ def test_particles_parity(wetting, lrotate, variant):
 
 
 @pytest.mark.parametrize("variant", [KERNEL_FUSED, KERNEL_TWOPASS])
+def test_particles_in_contact_parity(variant):
+    """Two spheres driven head-on into Hertz contact: their balls then hold
+    each other's surface nodes, so each link carries the moving-wall terms
+    and momentum shares of both particles (partBB_CG visits every particle of
+    the node's tile list, kernels_fluid_PART.CUF:916-973)."""
+    print(f"particles in contact variant={variant}", flush=True)
+    n = 40
+    cfg = default_config(n, sigma=0.02, beta=0.7, with_particles=1, n_particles=2, lrotate=1,
+                         max_vel=1.0, wetting=WET_NEUTRAL)
+    pos = np.zeros((13, 2), order="F")
+    pos[0:3, 0] = (14.3, 20.2, 19.8)
+    pos[0:3, 1] = (26.5, 20.6, 20.1)
+    pos[3, 0], pos[3, 1] = 0.05, -0.05
+    pos[7, 0], pos[6, 1] = 0.01, -0.01
+    pos[9, :] = 1.0
+    rr, rb = wl.spinodal_rho((n, n, n), seed=12)
+    g, o = both(cfg, INIT_RHO, pos=pos, rho_r=rr, rho_b=rb, variant=variant)
+    closest = 1e9
+    for _ in range(4):
+        assert g.step(10) == 0 and o.step(10) == 0
+        fl = o.flags()
+        assert (g.flags() == fl).all()
+        fluid = (fl == 1)[..., None].repeat(19, -1)
+        for c in (0, 1):
+            assert maxdiff(g.pops(c), o.pops(c), fluid) < 2e-5
+        gp, gf = g.particles()
+        op, of = o.particles()
+        assert maxdiff(gp[0:13], op[0:13]) < 1e-6
+        assert np.all(np.abs(gf - of) <= 1e-4 * (1 + np.abs(of)))
+        closest = min(closest, float(np.linalg.norm(op[0:3, 0] - op[0:3, 1])))
+    assert closest < 11.0   # the balls (r < 6) reached each other's surfaces
+
+
+@pytest.mark.parametrize("variant", [KERNEL_FUSED, KERNEL_TWOPASS])
 def test_prescribed_velocity_sphere_parity(variant):
     """A sphere with 1/m = 0 keeps its velocity (config: translating sphere):
     fill/erase every few steps, drag history vs the oracle."""
