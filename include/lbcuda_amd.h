@@ -206,6 +206,24 @@ int lbm_kernel_info(lbm_ctx* ctx, char* name, int32_t name_len,
 int lbm_fill_random_rho(uint64_t seed, double mean, double amp, int32_t nx, int32_t ny,
                         int32_t nz, float* rho_r, float* rho_b);
 
+/* Ensemble of independent boxes (no reference counterpart: the reference
+ * runs one box per process).  Boxes may differ in size, particles and
+ * parameters but share the storage format.  Each box is an ordinary lbm_ctx
+ * (initialise, set particles and read results through the calls above);
+ * lbm_ensemble_step advances all of them together: per step one persistent
+ * launch runs the fluid of every box from a shared work list of
+ * (box, tile, z chunk) items, and each particle kernel runs once for the
+ * particles of all boxes.  All boxes must be initialised and at the same
+ * step.  lbm_ensemble_autotune picks the tile and z chunk of the persistent
+ * launch (tiles without x wrap); lbm_ensemble_set_tile sets them. */
+typedef struct lbm_ensemble lbm_ensemble;
+int lbm_ensemble_create(int32_t nbox, const lbm_config* cfgs, lbm_ensemble** out);
+int lbm_ensemble_destroy(lbm_ensemble* ens);
+int lbm_ensemble_box(lbm_ensemble* ens, int32_t i, lbm_ctx** box);
+int lbm_ensemble_step(lbm_ensemble* ens, int32_t nsteps);
+int lbm_ensemble_set_tile(lbm_ensemble* ens, int32_t tile, int32_t zchunk);
+int lbm_ensemble_autotune(lbm_ensemble* ens, int32_t reps, int32_t* tile, int32_t* zchunk, double* best_ms);
+
 /* Device-to-device copy bandwidth in GB/s (read + write bytes over time) of
  * a plain copy kernel over two buffers of the given size, reps launches.
  * The reference probes host transfer bandwidth at start-up

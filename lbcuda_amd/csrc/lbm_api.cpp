@@ -383,6 +383,7 @@ int lbm_set_particles(lbm_ctx* c, int32_t n, const double* pos) {
     rc |= dalloc(&c->d_myt2, (size_t)3 * n);
     if (rc) { c->np = 0; return LBM_ERR_ALLOC; }
     c->np = n;
+    c->generation += 1;
   }
   /* read_input_atom (main.CUF:1300-1340): both levels start equal; forces
      start at the external force/torque, bounce-back parts at zero */
@@ -765,6 +766,202 @@ int lbm_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
   if (rc) return LBM_ERR_HIP;
   *gbs = 2.0 * (double)(n4 * 16) * reps / ((double)ms * 1e-3) / 1e9;
   return LBM_OK;
+}
+
+}  /* extern "C" */
+
+/* ---------------------------------------------------------------------- */
+/* Ensemble of independent boxes, advanced together: one persistent launch
+ * for the fluid of all boxes and one launch per particle kernel for all of
+ * their particles, on one stream (no reference counterpart: the reference
+ * runs one box per process). */
+struct lbm_ensemble {
+  std::vector<lbm_ctx*> box;
+  hipStream_t stream;
+  lbm_fluid_batch* fb;
+  lbm_part_batch* pb;
+  long gen;
+  int tile, zc;
+  int storage;
+};
+
+namespace {
+
+long ensemble_gen(const lbm_ensemble* e) {
+  long g = 0;
+  for (lbm_ctx* c : e->box) g += c->generation * 7 + c->np;
+  return g;
+}
+
+/* (re)build the per-box argument blocks when a box's buffers changed */
+int ensemble_prepare(lbm_ensemble* e) {
+  const long g = ensemble_gen(e);
+  if (e->fb && e->pb && g == e->gen) return 0;
+  lbm_fluid_batch_destroy(e->fb);
+  lbm_part_batch_destroy(e->pb);
+  e->fb = nullptr;
+  e->pb = nullptr;
+  int rc = 0;
+  e->fb = lbm_fluid_batch_create(e->box.data(), (int)e->box.size(), &rc);
+  if (!rc) e->pb = lbm_part_batch_create(e->box.data(), (int)e->box.size(), &rc);
+  if (!rc) rc = lbm_fluid_batch_config(e->fb, e->tile, e->zc);
+  if (rc) return rc;
+  e->gen = g;
+  return 0;
+}
+
+int ensemble_check_levels(const lbm_ensemble* e) {
+  for (lbm_ctx* c : e->box) {
+    if (!c->initialised) return LBM_ERR_STATE;
+    if (c->cur != e->box[0]->cur || c->step != e->box[0]->step) return LBM_ERR_BATCH;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_ensemble_create(int32_t nbox, const lbm_config* cfgs, lbm_ensemble** out) {
+  if (!out || !cfgs || nbox <= 0) return LBM_ERR_ARG;
+  *out = nullptr;
+  for (int i = 0; i < nbox; ++i)
+    if (cfgs[i].storage != cfgs[0].storage) return LBM_ERR_BATCH;
+  lbm_ensemble* e = new (std::nothrow) lbm_ensemble();
+  if (!e) return LBM_ERR_ALLOC;
+  e->fb = nullptr;
+  e->pb = nullptr;
+  e->gen = -1;
+  e->tile = 1;    /* 64 x 8 until lbm_ensemble_autotune / set_tile */
+  e->zc = 16;
+  e->storage = cfgs[0].storage;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return LBM_ERR_HIP;
+  }
+  for (int i = 0; i < nbox; ++i) {
+    lbm_ctx* c = nullptr;
+    const int rc = lbm_create(&cfgs[i], &c);
+    if (rc) {
+      lbm_ensemble_destroy(e);
+      return rc;
+    }
+    e->box.push_back(c);
+  }
+  *out = e;
+  return LBM_OK;
+}
+
+int lbm_ensemble_destroy(lbm_ensemble* e) {
+  if (!e) return LBM_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  lbm_fluid_batch_destroy(e->fb);
+  lbm_part_batch_destroy(e->pb);
+  for (lbm_ctx* c : e->box) lbm_destroy(c);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return LBM_OK;
+}
+
+int lbm_ensemble_box(lbm_ensemble* e, int32_t i, lbm_ctx** box) {
+  if (!e || !box || i < 0 || i >= (int)e->box.size()) return LBM_ERR_ARG;
+  *box = e->box[i];
+  return LBM_OK;
+}
+
+int lbm_ensemble_set_tile(lbm_ensemble* e, int32_t tile, int32_t zchunk) {
+  if (!e || !lbm_fluid_batch_tile_ok(tile) || zchunk < 1) return LBM_ERR_ARG;
+  e->tile = tile;
+  e->zc = zchunk;
+  if (e->fb) return lbm_fluid_batch_config(e->fb, tile, zchunk);
+  return LBM_OK;
+}
+
+int lbm_ensemble_step(lbm_ensemble* e, int32_t nsteps) {
+  if (!e || nsteps < 0) return LBM_ERR_ARG;
+  int rc = ensemble_check_levels(e);
+  if (rc) return rc;
+  for (lbm_ctx* c : e->box) rc |= hip_ok(hipStreamSynchronize(c->stream));
+  if (rc) return rc;
+  rc = ensemble_prepare(e);
+  if (rc) return rc;
+  int cur = e->box[0]->cur;
+  long step = e->box[0]->step;
+  for (int s = 0; s < nsteps && !rc; ++s) {
+    step += 1;
+    rc |= lbm_part_batch_pre(e->pb, cur, step, e->stream);
+    rc |= lbm_fluid_batch_launch(e->fb, cur, step, e->stream, nullptr);
+    rc |= lbm_part_batch_post(e->pb, cur, step, e->stream);
+    cur = 1 - cur;
+  }
+  for (lbm_ctx* c : e->box) {
+    c->cur = cur;
+    c->step = step;
+  }
+  if (rc) return LBM_ERR_HIP;
+  /* first error of any box */
+  std::vector<lbm::DevStatus> st(e->box.size());
+  for (size_t i = 0; i < e->box.size(); ++i)
+    rc |= hip_ok(hipMemcpyAsync(&st[i], e->box[i]->d_status, sizeof(lbm::DevStatus), hipMemcpyDeviceToHost, e->stream));
+  rc |= hip_ok(hipStreamSynchronize(e->stream));
+  if (rc) return LBM_ERR_HIP;
+  for (auto& x : st)
+    if (x.code) return x.code;
+  return LBM_OK;
+}
+
+int lbm_ensemble_autotune(lbm_ensemble* e, int32_t reps, int32_t* tile_out, int32_t* zc_out, double* best_ms) {
+  if (!e || reps <= 0) return LBM_ERR_ARG;
+  int rc = ensemble_check_levels(e);
+  if (rc) return rc;
+  for (lbm_ctx* c : e->box) rc |= hip_ok(hipStreamSynchronize(c->stream));
+  if (rc) return rc;
+  rc = ensemble_prepare(e);
+  if (rc) return rc;
+  /* scratch output levels: the timed launches leave every box as it was */
+  std::vector<void*> scratch(e->box.size(), nullptr);
+  for (size_t i = 0; i < e->box.size() && !rc; ++i)
+    if (hipMalloc(&scratch[i], e->box[i]->esize * 38 * e->box[i]->N) != hipSuccess) rc = LBM_ERR_ALLOC;
+  const int par = e->box[0]->cur;
+  lbm::FluidArgs* d_sa = rc ? nullptr : lbm_fluid_batch_scratch_args(e->fb, par, scratch.data());
+  if (!rc && !d_sa) rc = LBM_ERR_ALLOC;
+  std::vector<lbm::DevStatus> saved(e->box.size());
+  for (size_t i = 0; i < e->box.size() && !rc; ++i)
+    rc |= hip_ok(hipMemcpy(&saved[i], e->box[i]->d_status, sizeof(lbm::DevStatus), hipMemcpyDeviceToHost));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) rc |= hip_ok(hipEventCreate(&e0)) | hip_ok(hipEventCreate(&e1));
+  const int zcs[] = {8, 16, 32, 64};
+  double best = 1e30;
+  int bt = e->tile, bz = e->zc;
+  for (int t = 0; !rc && t < lbm_fused_num_tiles(); ++t) {
+    if (!lbm_fluid_batch_tile_ok(t)) continue;
+    for (int zc : zcs) {
+      rc |= lbm_fluid_batch_config(e->fb, t, zc);
+      rc |= lbm_fluid_batch_launch(e->fb, par, e->box[0]->step, e->stream, d_sa);   /* warm-up */
+      rc |= hip_ok(hipEventRecord(e0, e->stream));
+      for (int r = 0; r < reps; ++r) rc |= lbm_fluid_batch_launch(e->fb, par, e->box[0]->step, e->stream, d_sa);
+      rc |= hip_ok(hipEventRecord(e1, e->stream));
+      rc |= hip_ok(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      rc |= hip_ok(hipEventElapsedTime(&ms, e0, e1));
+      if (rc) break;
+      if (ms / reps < best) { best = ms / reps; bt = t; bz = zc; }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (size_t i = 0; i < e->box.size(); ++i)
+    if (!rc) rc |= hip_ok(hipMemcpy(e->box[i]->d_status, &saved[i], sizeof(lbm::DevStatus), hipMemcpyHostToDevice));
+  (void)hipFree(d_sa);
+  for (void* p : scratch) (void)hipFree(p);
+  if (rc) return rc;
+  e->tile = bt;
+  e->zc = bz;
+  rc = lbm_fluid_batch_config(e->fb, bt, bz);
+  if (tile_out) *tile_out = bt;
+  if (zc_out) *zc_out = bz;
+  if (best_ms) *best_ms = best;
+  return rc;
 }
 
 }  /* extern "C" */

@@ -139,6 +139,7 @@ struct lbm_ctx {
   int cur;                /* index of the state buffer / current time level    */
   long step;
   bool initialised;
+  long generation;        /* bumped when device buffers are (re)allocated    */
   int host_err;
   long host_err_step;
   char host_msg[160];
@@ -177,6 +178,22 @@ struct lbm_ctx {
   double t_total_ms;
   long t_launches;
 };
+
+/* batched (ensemble) step: opaque per-phase state, lbm_fused.hip / lbm_particles.hip */
+struct lbm_fluid_batch;
+struct lbm_part_batch;
+lbm_fluid_batch* lbm_fluid_batch_create(lbm_ctx* const* boxes, int nbox, int* rc);
+void lbm_fluid_batch_destroy(lbm_fluid_batch* b);
+bool lbm_fluid_batch_tile_ok(int tile);
+int lbm_fluid_batch_config(lbm_fluid_batch* b, int tile, int zc);
+int lbm_fluid_batch_launch(lbm_fluid_batch* b, int parity, long step, hipStream_t st, lbm::FluidArgs* d_override);
+lbm::FluidArgs* lbm_fluid_batch_scratch_args(lbm_fluid_batch* b, int parity, void* const* scratch);
+int lbm_fluid_batch_items(const lbm_fluid_batch* b);
+lbm_part_batch* lbm_part_batch_create(lbm_ctx* const* boxes, int nbox, int* rc);
+void lbm_part_batch_destroy(lbm_part_batch* b);
+int lbm_part_batch_count(const lbm_part_batch* b);
+int lbm_part_batch_pre(lbm_part_batch* b, int parity, long step, hipStream_t st);
+int lbm_part_batch_post(lbm_part_batch* b, int parity, long step, hipStream_t st);
 
 /* records the start/stop events around one timed launch (lbm_api.cpp) */
 void lbm_timing_begin(lbm_ctx* c);

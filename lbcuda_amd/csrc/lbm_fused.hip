@@ -39,6 +39,9 @@
 
 #include <math.h>
 
+#include <algorithm>
+#include <vector>
+
 using namespace lbm;
 
 namespace {
@@ -93,27 +96,17 @@ struct TileGeom {
   __device__ static constexpr int col(int c) { return WX ? (c < 0 ? c + TX : (c >= TX ? c - TX : c)) : c + 1; }
 };
 
+/* one TX x TY column of the box over the z chunk izc (planes [izc zc, izc zc + zc)) */
 template <class S, bool PART, int TX, int TY, bool WX>
-__global__ __launch_bounds__(TX * TY) void k_fused(FluidArgs a, int ntx, int nty, int nzc, int zc) {
+__device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity, int izc, int zc,
+                                           float (&sphi)[3][TY + 2][TileGeom<TX, TY, WX>::LW]) {
   using G = TileGeom<TX, TY, WX>;
-  __shared__ float sphi[3][TY + 2][G::LW];
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t N = a.N;
   const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
   const float rho0 = a.rho0;
   const S* src = (const S*)a.src;
   S* dst = (S*)a.dst;
-
-  /* tile id: XCD k (= blockIdx.x mod 8) gets the contiguous tiles [k q, (k+1) q) */
-  const int nb = ntx * nty * nzc;
-  int t = blockIdx.x;
-  {
-    const int q = nb >> 3;
-    if (t < (q << 3)) t = (t & 7) * q + (t >> 3);
-  }
-  const int itx = t % ntx;
-  const int ity = (t / ntx) % nty;
-  const int izc = t / (ntx * nty);
   const int x0 = itx * TX, y0 = ity * TY, z0 = izc * zc;
   const int z1 = min(z0 + zc, nz);
   const int w = WX ? TX : min(TX, nx - x0);
@@ -295,6 +288,49 @@ __global__ __launch_bounds__(TX * TY) void k_fused(FluidArgs a, int ntx, int nty
   }
 }
 
+template <class S, bool PART, int TX, int TY, bool WX>
+__global__ __launch_bounds__(TX * TY) void k_fused(FluidArgs a, int ntx, int nty, int nzc, int zc) {
+  __shared__ float sphi[3][TY + 2][TileGeom<TX, TY, WX>::LW];
+  /* tile id: XCD k (= blockIdx.x mod 8) gets the contiguous tiles [k q, (k+1) q) */
+  const int nb = ntx * nty * nzc;
+  int t = blockIdx.x;
+  {
+    const int q = nb >> 3;
+    if (t < (q << 3)) t = (t & 7) * q + (t >> 3);
+  }
+  const int itx = t % ntx;
+  const int ity = (t / ntx) % nty;
+  const int izc = t / (ntx * nty);
+  fused_tile<S, PART, TX, TY, WX>(a, itx, ity, izc, zc, sphi);
+}
+
+/* Batched step: many independent boxes in one persistent launch.  Work items
+ * are (box, tile x, tile y, z chunk), largest boxes first; each workgroup
+ * takes the next item from a device counter until the list is exhausted, so
+ * uneven boxes balance across the CUs.  The counter is never reset: launch L
+ * owns the range [base, base + nitems + gridDim.x), every workgroup ending
+ * with exactly one failed fetch. */
+template <class S, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_fused_batch(const FluidArgs* __restrict__ boxes,
+                                                         const int4* __restrict__ items, int nitems, int zc,
+                                                         unsigned long long* counter, unsigned long long base,
+                                                         int step) {
+  __shared__ float sphi[3][TY + 2][TileGeom<TX, TY, false>::LW];
+  __shared__ int next;
+  const int tid = threadIdx.x + TX * threadIdx.y;
+  for (;;) {
+    if (tid == 0) next = (int)(atomicAdd(counter, 1ull) - base);
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(next);
+    if (i >= nitems) break;
+    const int4 it = items[i];
+    FluidArgs a = boxes[it.x];
+    a.step = step;
+    fused_tile<S, true, TX, TY, false>(a, it.y, it.z, it.w, zc, sphi);
+    __syncthreads();
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* tile table: the autotuner picks one entry and a z-chunk length */
 struct TileEntry {
@@ -392,3 +428,157 @@ int lbm_launch_fused(lbm_ctx* c) {
   lbm_timing_end(c);
   return rc;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Fluid step of many boxes in one persistent launch.                      */
+struct lbm_fluid_batch {
+  int nbox;
+  int storage;
+  FluidArgs* d_args[2];           /* per box, for the two time-level parities */
+  std::vector<FluidArgs> h_args[2];
+  int4* d_items;
+  int nitems;
+  int tile, zc, grid;
+  unsigned long long* d_counter;
+  unsigned long long base;
+  std::vector<int> vol;           /* nodes per box, to order the work items */
+  std::vector<int> dims;          /* nx, ny, nz per box */
+};
+
+namespace {
+
+template <class S, int TX, int TY>
+int batch_grid() {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_batch<S, TX, TY>, TX * TY, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+  return per_cu * cus;
+}
+
+template <class S, int TX, int TY>
+void batch_launch(const lbm_fluid_batch* b, const FluidArgs* args, int step, hipStream_t st) {
+  hipLaunchKernelGGL((k_fused_batch<S, TX, TY>), dim3((unsigned)b->grid), dim3(TX, TY), 0, st, args, b->d_items,
+                     b->nitems, b->zc, b->d_counter, b->base, step);
+}
+
+template <class S>
+int batch_dispatch(lbm_fluid_batch* b, const FluidArgs* args, int step, hipStream_t st, bool grid_only) {
+  switch (b->tile) {
+    case 0: if (grid_only) return batch_grid<S, 64, 4>(); batch_launch<S, 64, 4>(b, args, step, st); break;
+    case 1: if (grid_only) return batch_grid<S, 64, 8>(); batch_launch<S, 64, 8>(b, args, step, st); break;
+    case 2: if (grid_only) return batch_grid<S, 64, 16>(); batch_launch<S, 64, 16>(b, args, step, st); break;
+    case 3: if (grid_only) return batch_grid<S, 128, 4>(); batch_launch<S, 128, 4>(b, args, step, st); break;
+    default: if (grid_only) return batch_grid<S, 128, 8>(); batch_launch<S, 128, 8>(b, args, step, st); break;
+  }
+  return 0;
+}
+
+int batch_call(lbm_fluid_batch* b, const FluidArgs* args, int step, hipStream_t st, bool grid_only) {
+  switch (b->storage) {
+    case LBM_STORE_F32: return batch_dispatch<float>(b, args, step, st, grid_only);
+    case LBM_STORE_F16: return batch_dispatch<__half>(b, args, step, st, grid_only);
+    default: return batch_dispatch<__hip_bfloat16>(b, args, step, st, grid_only);
+  }
+}
+
+}  // namespace
+
+/* tiles the batched kernel is built for: the entries of the tile table
+ * without x wrap (boxes of one batch differ in nx) */
+bool lbm_fluid_batch_tile_ok(int tile) { return tile >= 0 && tile <= 4; }
+
+lbm_fluid_batch* lbm_fluid_batch_create(lbm_ctx* const* boxes, int nbox, int* rc) {
+  *rc = 0;
+  lbm_fluid_batch* b = new lbm_fluid_batch();
+  b->nbox = nbox;
+  b->storage = boxes[0]->storage;
+  b->d_args[0] = b->d_args[1] = nullptr;
+  b->d_items = nullptr;
+  b->d_counter = nullptr;
+  b->nitems = 0;
+  b->base = 0;
+  b->tile = -1;
+  for (int i = 0; i < nbox; ++i) {
+    lbm_ctx* c = boxes[i];
+    const int cur = c->cur;
+    for (int par = 0; par < 2; ++par) {
+      c->cur = par;
+      b->h_args[par].push_back(make_fargs(c));
+    }
+    c->cur = cur;
+    b->vol.push_back((int)c->N);
+    b->dims.push_back(c->nx); b->dims.push_back(c->ny); b->dims.push_back(c->nz);
+  }
+  for (int par = 0; par < 2 && !*rc; ++par) {
+    if (hipMalloc(&b->d_args[par], sizeof(FluidArgs) * nbox) != hipSuccess) { *rc = LBM_ERR_ALLOC; break; }
+    if (hipMemcpy(b->d_args[par], b->h_args[par].data(), sizeof(FluidArgs) * nbox, hipMemcpyHostToDevice) != hipSuccess)
+      *rc = LBM_ERR_HIP;
+  }
+  if (!*rc && hipMalloc(&b->d_counter, sizeof(unsigned long long)) != hipSuccess) *rc = LBM_ERR_ALLOC;
+  if (!*rc && hipMemset(b->d_counter, 0, sizeof(unsigned long long)) != hipSuccess) *rc = LBM_ERR_HIP;
+  return b;
+}
+
+void lbm_fluid_batch_destroy(lbm_fluid_batch* b) {
+  if (!b) return;
+  for (int par = 0; par < 2; ++par) (void)hipFree(b->d_args[par]);
+  (void)hipFree(b->d_items);
+  (void)hipFree(b->d_counter);
+  delete b;
+}
+
+/* work items of tile `tile` and z chunk zc: (box, tile x, tile y, chunk),
+ * boxes in decreasing size so the big ones start first */
+int lbm_fluid_batch_config(lbm_fluid_batch* b, int tile, int zc) {
+  if (!lbm_fluid_batch_tile_ok(tile) || zc < 1) return LBM_ERR_ARG;
+  const TileEntry& e = kTiles[tile];
+  std::vector<int> order(b->nbox);
+  for (int i = 0; i < b->nbox; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int u, int v) { return b->vol[u] > b->vol[v]; });
+  std::vector<int4> items;
+  for (int i : order) {
+    const int nx = b->dims[3 * i], ny = b->dims[3 * i + 1], nz = b->dims[3 * i + 2];
+    const int ntx = (nx + e.tx - 1) / e.tx, nty = (ny + e.ty - 1) / e.ty, nzc = (nz + zc - 1) / zc;
+    for (int k = 0; k < nzc; ++k)
+      for (int j = 0; j < nty; ++j)
+        for (int t = 0; t < ntx; ++t) items.push_back(make_int4(i, t, j, k));
+  }
+  (void)hipFree(b->d_items);
+  b->d_items = nullptr;
+  if (hipMalloc(&b->d_items, sizeof(int4) * items.size()) != hipSuccess) return LBM_ERR_ALLOC;
+  if (hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return LBM_ERR_HIP;
+  b->nitems = (int)items.size();
+  b->tile = tile;
+  b->zc = zc;
+  b->grid = batch_call(b, nullptr, 0, nullptr, true);
+  return LBM_OK;
+}
+
+/* one launch for the fluid of every box; dst_override (per box, or NULL)
+ * redirects the output level (the autotuner times into scratch levels) */
+int lbm_fluid_batch_launch(lbm_fluid_batch* b, int parity, long step, hipStream_t st, FluidArgs* d_override) {
+  if (b->tile < 0) return LBM_ERR_STATE;
+  batch_call(b, d_override ? d_override : b->d_args[parity], (int)step, st, false);
+  b->base += (unsigned long long)b->nitems + (unsigned long long)b->grid;
+  return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
+}
+
+/* per-box argument blocks with the output level replaced (autotuning) */
+FluidArgs* lbm_fluid_batch_scratch_args(lbm_fluid_batch* b, int parity, void* const* scratch) {
+  std::vector<FluidArgs> a = b->h_args[parity];
+  for (int i = 0; i < b->nbox; ++i) a[i].dst = scratch[i];
+  FluidArgs* d = nullptr;
+  if (hipMalloc(&d, sizeof(FluidArgs) * b->nbox) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, a.data(), sizeof(FluidArgs) * b->nbox, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  return d;
+}
+
+int lbm_fluid_batch_items(const lbm_fluid_batch* b) { return b->nitems; }

@@ -18,6 +18,8 @@
 
 #include <math.h>
 
+#include <vector>
+
 using namespace lbm;
 
 namespace {
@@ -216,8 +218,7 @@ __device__ __forceinline__ void belt_rho(const PartArgs& a, size_t m, float* rr,
 }
 
 /* ---------------- flags ------------------------------------------------ */
-__global__ __launch_bounds__(TPB) void k_unstamp(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_unstamp_body(const PartArgs& a, int p) {
   const int cx = a.stamp_cur[3 * p], cy = a.stamp_cur[3 * p + 1], cz = a.stamp_cur[3 * p + 2];
   if (cx == INT32_MIN) return;
   for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
@@ -227,8 +228,7 @@ __global__ __launch_bounds__(TPB) void k_unstamp(PartArgs a) {
   }
 }
 
-__global__ __launch_bounds__(TPB) void k_stamp(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_stamp_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
   for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
@@ -237,8 +237,7 @@ __global__ __launch_bounds__(TPB) void k_stamp(PartArgs a) {
   }
 }
 
-__global__ __launch_bounds__(TPB) void k_resolve(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_resolve_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
   for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
@@ -255,8 +254,7 @@ __global__ __launch_bounds__(TPB) void k_resolve(PartArgs a) {
 
 /* ---------------- erase: fluid -> solid (del_fluid, PART:383-561) -------- */
 template <class S>
-__global__ __launch_bounds__(TPB) void k_erase(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
@@ -312,8 +310,7 @@ __device__ bool fill_candidate(const PartArgs& a, size_t n) {
   return (fo == FL_LIST || fo == FL_DEAD) && a.fl_cur[n] == FL_FLUID;
 }
 
-__global__ __launch_bounds__(TPB) void k_fill_claim(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_fill_claim_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
@@ -324,8 +321,7 @@ __global__ __launch_bounds__(TPB) void k_fill_claim(PartArgs a) {
 }
 
 template <class S>
-__global__ __launch_bounds__(TPB) void k_fill_apply(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_fill_apply_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
@@ -394,8 +390,7 @@ __global__ __launch_bounds__(TPB) void k_fill_apply(PartArgs a) {
   }
 }
 
-__global__ __launch_bounds__(TPB) void k_fill_release(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_fill_release_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
@@ -407,8 +402,7 @@ __global__ __launch_bounds__(TPB) void k_fill_release(PartArgs a) {
 
 /* ---------------- wetting of surface nodes -------------------------------- */
 template <class S>
-__global__ __launch_bounds__(TPB) void k_wetting(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
   for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
@@ -489,8 +483,7 @@ __global__ __launch_bounds__(TPB) void k_wetting(PartArgs a) {
  * kernels_fluid_CG.CUF:398-509).  Offsets: the template and the D3Q27 ring
  * of its surface nodes. */
 template <class S>
-__global__ __launch_bounds__(TPB) void k_shell(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_shell_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
   /* the particle's own nodes: MINDENS (every template node is non-fluid) */
@@ -620,8 +613,7 @@ __device__ __forceinline__ void link_force(const PartArgs& a, const NbrCache& nc
 }
 
 template <class S>
-__global__ __launch_bounds__(TPB) void k_links(PartArgs a) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
   __shared__ NbrCache nc;
   __shared__ int self_slot;
   const int nnb = min(a.nnbr[p], MAXNB);
@@ -711,8 +703,7 @@ __global__ __launch_bounds__(TPB) void k_links(PartArgs a) {
  * order), so k_links finds every particle whose ball reaches i's surface.
  * More than MAXNB raises LBM_ERR_LIST_OVERFLOW, as makeListAtomsGPU raises
  * on a full tile list (kernels_fluid_PART.CUF:2436-2439). */
-__global__ __launch_bounds__(TPB) void k_neigh(PartArgs a) {
-  const int i = blockIdx.x;
+__device__ __forceinline__ void k_neigh_body(const PartArgs& a, int i) {
   int ci[3];
   centre(a.pos_cur, i, ci);
   __shared__ int wcount[TPB / 64];
@@ -762,10 +753,9 @@ struct PairParams {
   int hertz, lub;
 };
 
-__global__ __launch_bounds__(TPB) void k_pairs(PartArgs a, PairParams q) {
+__device__ __forceinline__ void k_pairs_body(const PartArgs& a, int i, const PairParams& q) {
   /* one workgroup per particle i, its threads over the partners j; FP64
      partial forces reduced in a fixed order */
-  const int i = blockIdx.x;
   double f[6] = {0, 0, 0, 0, 0, 0};
   const double xi = a.pos_cur[13 * i], yi = a.pos_cur[13 * i + 1], zi = a.pos_cur[13 * i + 2];
   for (int j = threadIdx.x; j < a.np; j += TPB) {
@@ -806,9 +796,7 @@ struct IntParams {
   double ext_f[3], ext_t[3];
 };
 
-__global__ __launch_bounds__(TPB) void k_integrate(PartArgs a, IntParams q) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
+__device__ __forceinline__ void k_integrate_body(const PartArgs& a, int i, const IntParams& q) {
   const double* P = a.pos_cur + 13 * i;
   double* Pn = a.pos_new + 13 * i;
   double* Fc = a.f_cur + 12 * i;
@@ -914,6 +902,71 @@ __global__ __launch_bounds__(TPB) void k_integrate(PartArgs a, IntParams q) {
       a.myt[3 * i + d] = 0.0;
       a.myt2[3 * i + d] = q.ext_t[d];
     }
+}
+
+/* ---------------- kernels: one box, and many boxes per launch ------------
+ * Single box: workgroup (or thread) p handles particle p of the box.
+ * Batched: item g = (box, particle) over all boxes; the per-box argument
+ * blocks sit in device memory for both time-level parities. */
+struct BatchStep {
+  int step;
+  int lag_from_pops;
+};
+
+__device__ __forceinline__ PartArgs batch_args(const PartArgs* __restrict__ args, int box, BatchStep b) {
+  PartArgs a = args[box];
+  a.step = b.step;
+  a.lag_from_pops = b.lag_from_pops;
+  return a;
+}
+
+#define LBM_PART_KERNEL(NAME)                                                                         \
+  __global__ __launch_bounds__(TPB) void NAME(PartArgs a) { NAME##_body(a, blockIdx.x); }          \
+  __global__ __launch_bounds__(TPB) void NAME##_batch(const PartArgs* __restrict__ args,           \
+                                                      const int2* __restrict__ items, BatchStep b) { \
+    const int2 it = items[blockIdx.x];                                                                \
+    NAME##_body(batch_args(args, it.x, b), it.y);                                                     \
+  }
+#define LBM_PART_KERNEL_S(NAME)                                                                       \
+  template <class S>                                                                                  \
+  __global__ __launch_bounds__(TPB) void NAME(PartArgs a) { NAME##_body<S>(a, blockIdx.x); }       \
+  template <class S>                                                                                  \
+  __global__ __launch_bounds__(TPB) void NAME##_batch(const PartArgs* __restrict__ args,           \
+                                                      const int2* __restrict__ items, BatchStep b) { \
+    const int2 it = items[blockIdx.x];                                                                \
+    NAME##_body<S>(batch_args(args, it.x, b), it.y);                                                  \
+  }
+
+LBM_PART_KERNEL(k_unstamp)
+LBM_PART_KERNEL(k_stamp)
+LBM_PART_KERNEL(k_resolve)
+LBM_PART_KERNEL(k_fill_claim)
+LBM_PART_KERNEL(k_fill_release)
+LBM_PART_KERNEL(k_neigh)
+LBM_PART_KERNEL_S(k_erase)
+LBM_PART_KERNEL_S(k_fill_apply)
+LBM_PART_KERNEL_S(k_wetting)
+LBM_PART_KERNEL_S(k_shell)
+LBM_PART_KERNEL_S(k_links)
+
+__global__ __launch_bounds__(TPB) void k_pairs(PartArgs a, PairParams q) { k_pairs_body(a, blockIdx.x, q); }
+__global__ __launch_bounds__(TPB) void k_pairs_batch(const PartArgs* __restrict__ args,
+                                                     const PairParams* __restrict__ qs,
+                                                     const int2* __restrict__ items, BatchStep b) {
+  const int2 it = items[blockIdx.x];
+  k_pairs_body(batch_args(args, it.x, b), it.y, qs[it.x]);
+}
+__global__ __launch_bounds__(TPB) void k_integrate(PartArgs a, IntParams q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.np) k_integrate_body(a, i, q);
+}
+__global__ __launch_bounds__(TPB) void k_integrate_batch(const PartArgs* __restrict__ args,
+                                                         const IntParams* __restrict__ qs,
+                                                         const int2* __restrict__ items, int nitems, BatchStep b) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nitems) return;
+  const int2 it = items[g];
+  k_integrate_body(batch_args(args, it.x, b), it.y, qs[it.x]);
 }
 
 PartArgs make_pargs(lbm_ctx* c) {
@@ -1026,8 +1079,8 @@ int lbm_particles_bounce(lbm_ctx* c) {
 
 int lbm_particles_max_neighbours() { return MAXNB; }
 
-int lbm_particles_integrate(lbm_ctx* c) {
-  PartArgs a = make_pargs(c);
+namespace {
+PairParams make_pair_params(const lbm_ctx* c) {
   const lbm_config& g = c->cfg;
   PairParams pp;
   pp.hz_k = g.hz_k; pp.hz_rmin = g.hz_rmin; pp.hz_rcap = g.hz_rcap;
@@ -1041,13 +1094,125 @@ int lbm_particles_integrate(lbm_ctx* c) {
   pp.visc = ((double)g.tau_r - 0.5) / 3.0;
   pp.hertz = g.hertz;
   pp.lub = g.lubrication;
-  unsigned nb = (unsigned)((c->np + TPB - 1) / TPB);
-  hipLaunchKernelGGL(k_pairs, dim3(c->np), dim3(TPB), 0, c->stream, a, pp);
+  return pp;
+}
+
+IntParams make_int_params(const lbm_ctx* c) {
+  const lbm_config& g = c->cfg;
+  const double R = g.radius;
   IntParams ip;
   ip.rmass = g.rmass;
   ip.rotin = g.rmass > 0 ? 2.0 / 5.0 * (1.0 / (double)g.rmass) * R * R : 0.0;
   ip.max_vel = g.max_vel;
   for (int d = 0; d < 3; ++d) { ip.ext_f[d] = g.ext_force[d]; ip.ext_t[d] = g.lrotate ? g.ext_torque[d] : 0.0; }
+  return ip;
+}
+}  // namespace
+
+int lbm_particles_integrate(lbm_ctx* c) {
+  PartArgs a = make_pargs(c);
+  const PairParams pp = make_pair_params(c);
+  const IntParams ip = make_int_params(c);
+  unsigned nb = (unsigned)((c->np + TPB - 1) / TPB);
+  hipLaunchKernelGGL(k_pairs, dim3(c->np), dim3(TPB), 0, c->stream, a, pp);
   hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(TPB), 0, c->stream, a, ip);
+  return last_err();
+}
+
+/* ---------------------------------------------------------------------- */
+/* Particle phases of many boxes, one launch per kernel for all of them.   */
+struct lbm_part_batch {
+  int nbox;
+  int storage;
+  PartArgs* d_args[2];     /* per box, for the two time-level parities */
+  PairParams* d_pp;
+  IntParams* d_ip;
+  int2* d_items;           /* (box, particle) */
+  int nitems;
+};
+
+lbm_part_batch* lbm_part_batch_create(lbm_ctx* const* boxes, int nbox, int* rc) {
+  *rc = 0;
+  lbm_part_batch* b = new lbm_part_batch();
+  b->nbox = nbox;
+  b->storage = boxes[0]->storage;
+  std::vector<PartArgs> args[2];
+  std::vector<PairParams> pp(nbox);
+  std::vector<IntParams> ip(nbox);
+  std::vector<int2> items;
+  for (int i = 0; i < nbox; ++i) {
+    lbm_ctx* c = boxes[i];
+    const int cur = c->cur;
+    for (int par = 0; par < 2; ++par) {
+      c->cur = par;
+      args[par].push_back(make_pargs(c));
+      args[par].back().rho_from_pops = 1;   /* the batch always runs the single-pass fluid kernel */
+    }
+    c->cur = cur;
+    pp[i] = make_pair_params(c);
+    ip[i] = make_int_params(c);
+    for (int q = 0; q < c->np; ++q) items.push_back(make_int2(i, q));
+  }
+  b->nitems = (int)items.size();
+  b->d_args[0] = b->d_args[1] = nullptr;
+  b->d_pp = nullptr; b->d_ip = nullptr; b->d_items = nullptr;
+  auto up = [&](void** dptr, const void* h, size_t bytes) {
+    if (*rc) return;
+    if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) { *rc = LBM_ERR_ALLOC; return; }
+    if (bytes && hipMemcpy(*dptr, h, bytes, hipMemcpyHostToDevice) != hipSuccess) *rc = LBM_ERR_HIP;
+  };
+  for (int par = 0; par < 2; ++par) up((void**)&b->d_args[par], args[par].data(), sizeof(PartArgs) * nbox);
+  up((void**)&b->d_pp, pp.data(), sizeof(PairParams) * nbox);
+  up((void**)&b->d_ip, ip.data(), sizeof(IntParams) * nbox);
+  up((void**)&b->d_items, items.data(), sizeof(int2) * items.size());
+  return b;
+}
+
+void lbm_part_batch_destroy(lbm_part_batch* b) {
+  if (!b) return;
+  for (int par = 0; par < 2; ++par) (void)hipFree(b->d_args[par]);
+  (void)hipFree(b->d_pp);
+  (void)hipFree(b->d_ip);
+  (void)hipFree(b->d_items);
+  delete b;
+}
+
+int lbm_part_batch_count(const lbm_part_batch* b) { return b->nitems; }
+
+/* flags, erase, shell densities, fill, wetting (the particle work before the
+ * collide-stream kernel, in lbm_step's order) */
+int lbm_part_batch_pre(lbm_part_batch* b, int parity, long step, hipStream_t st) {
+  if (b->nitems == 0) return 0;
+  const PartArgs* A = b->d_args[parity];
+  BatchStep bs;
+  bs.step = (int)step;
+  bs.lag_from_pops = step > 1 ? 1 : 0;
+  const dim3 g((unsigned)b->nitems), t(TPB);
+  hipLaunchKernelGGL(k_unstamp_batch, g, t, 0, st, A, b->d_items, bs);
+  hipLaunchKernelGGL(k_stamp_batch, g, t, 0, st, A, b->d_items, bs);
+  hipLaunchKernelGGL(k_resolve_batch, g, t, 0, st, A, b->d_items, bs);
+  LBM_STORAGE_SWITCH(b->storage,
+    hipLaunchKernelGGL(k_erase_batch<S>, g, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_shell_batch<S>, g, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_fill_claim_batch, g, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_fill_apply_batch<S>, g, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_fill_release_batch, g, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_wetting_batch<S>, g, t, 0, st, A, b->d_items, bs))
+  return last_err();
+}
+
+/* neighbour lists, bounce-back links, pair forces, rigid-body update */
+int lbm_part_batch_post(lbm_part_batch* b, int parity, long step, hipStream_t st) {
+  if (b->nitems == 0) return 0;
+  const PartArgs* A = b->d_args[parity];
+  BatchStep bs;
+  bs.step = (int)step;
+  bs.lag_from_pops = step > 1 ? 1 : 0;
+  const dim3 g((unsigned)b->nitems), t(TPB);
+  hipLaunchKernelGGL(k_neigh_batch, g, t, 0, st, A, b->d_items, bs);
+  LBM_STORAGE_SWITCH(b->storage, hipLaunchKernelGGL(k_links_batch<S>, g, t, 0, st, A, b->d_items, bs))
+  hipLaunchKernelGGL(k_pairs_batch, g, t, 0, st, A, b->d_pp, b->d_items, bs);
+  hipLaunchKernelGGL(k_integrate_batch, dim3((unsigned)((b->nitems + TPB - 1) / TPB)), t, 0, st, A, b->d_ip,
+                     b->d_items, b->nitems, bs);
   return last_err();
 }

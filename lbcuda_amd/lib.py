@@ -64,6 +64,12 @@ def lib():
     L.lbm_tile_shape.argtypes = [i32, P(i32), P(i32), P(i32)]
     L.lbm_autotune.argtypes = [vp, i32, P(i32), P(i32), P(ctypes.c_double)]
     L.lbm_copy_bandwidth.argtypes = [ctypes.c_int64, i32, P(ctypes.c_double)]
+    L.lbm_ensemble_create.argtypes = [i32, P(LbmConfig), P(vp)]
+    L.lbm_ensemble_destroy.argtypes = [vp]
+    L.lbm_ensemble_box.argtypes = [vp, i32, P(vp)]
+    L.lbm_ensemble_step.argtypes = [vp, i32]
+    L.lbm_ensemble_set_tile.argtypes = [vp, i32, i32]
+    L.lbm_ensemble_autotune.argtypes = [vp, i32, P(i32), P(i32), P(ctypes.c_double)]
     L.lbm_version.restype = ctypes.c_char_p
     L.lbm_fill_random_rho.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double, i32, i32, i32, vp, vp]
     L.lbm_fill_random_rho.restype = ctypes.c_int
@@ -71,7 +77,9 @@ def lib():
                  "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments", "lbm_get_flags",
                  "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
                  "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_set_kernel", "lbm_get_kernel",
-                 "lbm_tile_shape", "lbm_autotune", "lbm_copy_bandwidth"):
+                 "lbm_tile_shape", "lbm_autotune", "lbm_copy_bandwidth", "lbm_ensemble_create",
+                 "lbm_ensemble_destroy", "lbm_ensemble_box", "lbm_ensemble_step", "lbm_ensemble_set_tile",
+                 "lbm_ensemble_autotune"):
         getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -82,7 +90,8 @@ EXPORTED = ["lbm_config_default", "lbm_create", "lbm_destroy", "lbm_init_fields"
             "lbm_get_flags", "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
             "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version", "lbm_fill_random_rho",
             "lbm_set_kernel", "lbm_get_kernel", "lbm_num_tiles", "lbm_tile_shape", "lbm_autotune",
-            "lbm_copy_bandwidth"]
+            "lbm_copy_bandwidth", "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_box",
+            "lbm_ensemble_step", "lbm_ensemble_set_tile", "lbm_ensemble_autotune"]
 
 
 def _f32(a):
@@ -96,17 +105,21 @@ def _ptr(a):
 class Box:
     """One periodic box on the GPU."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, handle=None):
         self.cfg = cfg
         self.n = (cfg.nx, cfg.ny, cfg.nz)
         self.h = ctypes.c_void_p()
-        self._check(lib().lbm_create(ctypes.byref(cfg), ctypes.byref(self.h)), "lbm_create")
+        self.owned = handle is None
+        if handle is None:
+            self._check(lib().lbm_create(ctypes.byref(cfg), ctypes.byref(self.h)), "lbm_create")
+        else:
+            self.h = handle
         self.npart = 0
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             lib().lbm_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __enter__(self):
         return self
@@ -221,6 +234,58 @@ class Box:
         b = ctypes.c_double()
         self._check(lib().lbm_kernel_info(self.h, buf, 128, ctypes.byref(b)), "lbm_kernel_info")
         return buf.value.decode(), b.value
+
+
+class Ensemble:
+    """Independent boxes advanced together (lbm_ensemble_*): the fluid of all
+    boxes in one persistent launch per step, each particle kernel once for
+    the particles of all boxes.  boxes[i] is a Box view of box i (initialise
+    and read it through the Box calls; it is freed with the ensemble)."""
+
+    def __init__(self, cfgs):
+        arr = (LbmConfig * len(cfgs))(*cfgs)
+        self.h = ctypes.c_void_p()
+        rc = lib().lbm_ensemble_create(len(cfgs), arr, ctypes.byref(self.h))
+        if rc:
+            raise LbmError(rc, "lbm_ensemble_create")
+        self.boxes = []
+        for i, c in enumerate(cfgs):
+            bh = ctypes.c_void_p()
+            rc = lib().lbm_ensemble_box(self.h, i, ctypes.byref(bh))
+            if rc:
+                raise LbmError(rc, "lbm_ensemble_box")
+            self.boxes.append(Box(c, handle=bh))
+
+    def step(self, n=1):
+        rc = lib().lbm_ensemble_step(self.h, n)
+        if rc in (1, 2, 3, 9, 10):
+            raise LbmError(rc, "lbm_ensemble_step")
+        return rc
+
+    def set_tile(self, tile, zchunk):
+        rc = lib().lbm_ensemble_set_tile(self.h, tile, zchunk)
+        if rc:
+            raise LbmError(rc, "lbm_ensemble_set_tile")
+
+    def autotune(self, reps=3):
+        t, z, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        rc = lib().lbm_ensemble_autotune(self.h, reps, ctypes.byref(t), ctypes.byref(z), ctypes.byref(ms))
+        if rc:
+            raise LbmError(rc, "lbm_ensemble_autotune")
+        return t.value, z.value, ms.value
+
+    def close(self):
+        if self.h:
+            for b in self.boxes:
+                b.h = ctypes.c_void_p()
+            lib().lbm_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fill_random_rho(shape, seed, mean=0.5, amp=0.05):

@@ -131,3 +131,29 @@ def pickering_inputs(n=256, nparticles=1280, seed=2024):
     rr, rb = spinodal_rho((n, n, n), seed=seed)
     pos = random_spheres((n, n, n), nparticles, 5.5, seed=seed + 1)
     return rr, rb, pos
+
+
+def ensemble_workload(nbox=64, seed=77, storage=0, edge_range=(48, 96), spheres_range=(2, 16),
+                      radius_range=(4.0, 8.0)):
+    """Batched ensemble of independent CG boxes: box edges uniform in
+    edge_range (each axis drawn separately, seeded), 2..16 spheres per box at
+    seeded non-overlapping positions, one sphere radius per box drawn from
+    radius_range (the sphere template is a per-box table).  Each box carries
+    its own 50/50 random colour field.  Returns (configs, inputs) with
+    inputs[i] = (rho_r, rho_b, pos)."""
+    cfgs, inputs = [], []
+    for b in range(nbox):
+        u = uniform01(seed * 1000003 + b, np.arange(6, dtype=np.uint64))
+        lo, hi = edge_range
+        nx, ny, nz = (int(lo + math.floor(u[k] * (hi - lo + 1))) for k in range(3))
+        ns = int(spheres_range[0] + math.floor(u[3] * (spheres_range[1] - spheres_range[0] + 1)))
+        radius = float(radius_range[0] + u[4] * (radius_range[1] - radius_range[0]))
+        cfg = default_config(nx, sigma=0.02, beta=0.7, storage=storage, with_particles=1, n_particles=ns,
+                             radius=radius, lrotate=1, max_vel=0.3, wetting=WET_NEUTRAL, hertz=1,
+                             lubrication=1, lub_k=1.0, lub_rmin=1.0, lub_rcap=0.1)
+        cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+        rr, rb = spinodal_rho((nx, ny, nz), seed=seed * 7919 + b)
+        pos = random_spheres((nx, ny, nz), ns, radius, seed=seed * 104729 + b)
+        cfgs.append(cfg)
+        inputs.append((rr, rb, pos))
+    return cfgs, inputs
