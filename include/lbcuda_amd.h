@@ -206,6 +206,37 @@ int lbm_kernel_info(lbm_ctx* ctx, char* name, int32_t name_len,
 int lbm_fill_random_rho(uint64_t seed, double mean, double amp, int32_t nx, int32_t ny,
                         int32_t nz, float* rho_r, float* rho_b);
 
+/* Learned near-contact closure (no reference counterpart; the reference's
+ * near-contact model is the lubrication term of addHertzian,
+ * kernels_fluid_PART.CUF:1463-1501, which stays the fallback).
+ * A 4 -> 64 -> 64 -> 3 MLP (tanh hidden layers, FP32) maps the pair features
+ * (gap/R, v_n, v_t, R) to the corrections (F_n, F_t, T):
+ *   gap = r - 2R (r = centre distance), n = unit vector from i to j,
+ *   u = v_i - v_j + R (omega_i + omega_j) x n, v_n = u.n (> 0 approaching),
+ *   t = unit vector of u - v_n n, v_t = |u - v_n n|;
+ *   force on i += -F_n n - F_t t, torque on i += T (n x t).
+ * Parameters (LBM_MLP_NPARAM floats): W1[64][4] b1[64] W2[64][64] b2[64]
+ * W3[3][64] b3[3].  The network sees (features - in_mean) / in_scale and its
+ * outputs are multiplied by out_scale. */
+#define LBM_MLP_NPARAM 4675
+/* closure block: params, in_mean[4], in_scale[4], out_scale[3], lo[4], hi[4]
+ * (the feature box of the training data: outside it, or at gap >= gap_max,
+ * the lubrication term applies instead) and gap_max */
+#define LBM_CLOSURE_NFLOAT (LBM_MLP_NPARAM + 20)
+
+/* Batch inference on the GPU: x (n,4) normalised features -> y (n,3). */
+int lbm_mlp_forward(const float* params, const float* x, float* y, int32_t n);
+/* Mean squared error over the n samples and its gradient w.r.t. params. */
+int lbm_mlp_loss_grad(const float* params, const float* x, const float* t, int32_t n, double* loss, float* grad);
+/* Adam training on the GPU (forward, backward and update kernels of this
+ * library); params are read as the initial weights and overwritten.  Each
+ * epoch visits the samples in a seeded random order, `batch` at a time. */
+int lbm_mlp_train(float* params, const float* x, const float* t, int32_t n, int32_t epochs, int32_t batch, float lr,
+                  uint64_t seed, double* final_loss);
+/* Enable (block != NULL) or disable the closure in a box's pair forces;
+ * requires cfg.closure != 0 at create. */
+int lbm_set_closure(lbm_ctx* ctx, const float* block);
+
 /* Ensemble of independent boxes (no reference counterpart: the reference
  * runs one box per process).  Boxes may differ in size, particles and
  * parameters but share the storage format.  Each box is an ordinary lbm_ctx

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "liblbcuda_amd.so")
-SOURCES = ["lbm_fused.hip", "lbm_fluid.hip", "lbm_particles.hip", "lbm_api.cpp"]
+SOURCES = ["lbm_fused.hip", "lbm_fluid.hip", "lbm_particles.hip", "lbm_mlp.hip", "lbm_api.cpp"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 

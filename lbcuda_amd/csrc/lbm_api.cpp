@@ -205,7 +205,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
   /* shifted 16-bit storage: the fused kernel re-stores non-fluid values
      through float, exact while w_0 rho0 + |f - w rho0| < 0.5 */
   if (cfg->storage != LBM_STORE_F32 && !(cfg->store_rho0 >= 0.0f && cfg->store_rho0 <= 1.2f)) return LBM_ERR_ARG;
-  if (cfg->with_particles && (cfg->radius < 1.0f || cfg->radius > 30.0f)) return LBM_ERR_ARG;
+  /* template offsets are int8: radius + 3 must stay below 127 */
+  if (cfg->with_particles && (cfg->radius < 1.0f || cfg->radius > 60.0f)) return LBM_ERR_ARG;
   /* particle kernels wrap offsets of up to rmax + 2 nodes once per axis */
   if (cfg->with_particles) {
     const int rmx = (int)(cfg->radius + 1.0f) + 3;
@@ -283,7 +284,7 @@ int lbm_destroy(lbm_ctx* c) {
   dfree(c->d_status);
   dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
   dfree(c->d_claim); dfree(c->d_tmpl); dfree(c->d_ball); dfree(c->d_ring); dfree(c->d_tlist);
-  dfree(c->d_nbr); dfree(c->d_nnbr); dfree(c->d_contrib);
+  dfree(c->d_nbr); dfree(c->d_nnbr); dfree(c->d_contrib); dfree(c->d_closure);
   for (int e = 0; e < 2 * c->ev_cap; ++e) (void)hipEventDestroy(c->ev_pairs[e]);
   delete[] c->ev_pairs;
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -961,6 +962,34 @@ int lbm_ensemble_autotune(lbm_ensemble* e, int32_t reps, int32_t* tile_out, int3
   if (tile_out) *tile_out = bt;
   if (zc_out) *zc_out = bz;
   if (best_ms) *best_ms = best;
+  return rc;
+}
+
+}  /* extern "C" */
+
+extern "C" {
+
+int lbm_set_closure(lbm_ctx* c, const float* block) {
+  if (!c) return LBM_ERR_ARG;
+  if (!block) {
+    dfree(c->d_closure);
+    c->d_closure = nullptr;
+    c->generation += 1;
+    return LBM_OK;
+  }
+  if (!c->cfg.closure || !c->cfg.with_particles) return LBM_ERR_ARG;
+  const float* sc = block + LBM_MLP_NPARAM + 4;
+  for (int k = 0; k < 7; ++k)
+    if (!(sc[k] != 0.0f)) return LBM_ERR_ARG;   /* scales must be non-zero */
+  const float gmax = block[LBM_CLOSURE_NFLOAT - 1];
+  if (!(gmax > 0.0f && gmax < 10.0f)) return LBM_ERR_ARG;
+  if (!c->d_closure) {
+    int rc = dalloc(&c->d_closure, LBM_CLOSURE_NFLOAT);
+    if (rc) return rc;
+  }
+  int rc = hip_ok(hipMemcpy(c->d_closure, block, sizeof(float) * LBM_CLOSURE_NFLOAT, hipMemcpyHostToDevice));
+  c->closure_gap_max = gmax;
+  c->generation += 1;
   return rc;
 }
 

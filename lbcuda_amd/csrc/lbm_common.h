@@ -121,6 +121,39 @@ struct FluidArgs {
 
 }  // namespace lbm
 
+/* Near-contact closure network 4 -> 64 -> 64 -> 3 (tanh, tanh, linear),
+ * parameters W1[64][4] b1[64] W2[64][64] b2[64] W3[3][64] b3[3].  The same
+ * fused multiply-add order as the training forward pass in lbm_mlp.hip. */
+__device__ inline void lbm_mlp_eval(const float* __restrict__ prm, const float in[4], float out[3]) {
+  constexpr int NH = 64, OB1 = NH * 4, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH, OB3 = OW3 + 3 * NH;
+  float a1[NH];
+#pragma unroll 8
+  for (int j = 0; j < NH; ++j) {
+    float z = prm[OB1 + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) z = fmaf(prm[4 * j + i], in[i], z);
+    a1[j] = tanhf(z);
+  }
+  float y0 = prm[OB3], y1 = prm[OB3 + 1], y2 = prm[OB3 + 2];
+  float a2[NH];
+#pragma unroll 4
+  for (int k = 0; k < NH; ++k) {
+    float z = prm[OB2 + k];
+#pragma unroll 16
+    for (int j = 0; j < NH; ++j) z = fmaf(prm[OW2 + NH * k + j], a1[j], z);
+    a2[k] = tanhf(z);
+  }
+#pragma unroll 16
+  for (int k = 0; k < NH; ++k) y0 = fmaf(prm[OW3 + k], a2[k], y0);
+#pragma unroll 16
+  for (int k = 0; k < NH; ++k) y1 = fmaf(prm[OW3 + NH + k], a2[k], y1);
+#pragma unroll 16
+  for (int k = 0; k < NH; ++k) y2 = fmaf(prm[OW3 + 2 * NH + k], a2[k], y2);
+  out[0] = y0;
+  out[1] = y1;
+  out[2] = y2;
+}
+
 /* ---------------------------------------------------------------------- */
 /* internal context (host side) */
 struct lbm_ctx {
@@ -166,6 +199,9 @@ struct lbm_ctx {
   int8_t* d_ball;         /* (4, nball) offsets with r^2 <= rmax^2; [3]=1 if <   */
   int nball;
   int rmax;
+  /* near-contact closure: parameters + normalisation (LBM_CLOSURE_NFLOAT) */
+  float* d_closure;       /* NULL when disabled                                */
+  float closure_gap_max;
   /* collide-stream kernel: LBM_KERNEL_*, fused tile index and z chunk */
   int variant;
   int tile;

@@ -81,6 +81,7 @@ struct PartArgs {
   float radius;
   int lrotate;
   int wetting;
+  const float* closure;    /* learned near-contact block or NULL        */
   int rho_from_pops;       /* single-pass step: densities away from the
                               particle shells are not stored, sum them    */
   int lag_from_pops;       /* erase: lagging density from the other level */
@@ -218,10 +219,13 @@ __device__ __forceinline__ void belt_rho(const PartArgs& a, size_t m, float* rr,
 }
 
 /* ---------------- flags ------------------------------------------------ */
+/* Kernels without per-particle sums spread one particle's offset list over
+ * gridDim.y workgroups (blockIdx.y = chunk), so large spheres (4x resolved
+ * pair runs) do not serialise on one workgroup. */
 __device__ __forceinline__ void k_unstamp_body(const PartArgs& a, int p) {
   const int cx = a.stamp_cur[3 * p], cy = a.stamp_cur[3 * p + 1], cz = a.stamp_cur[3 * p + 2];
   if (cx == INT32_MIN) return;
-  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
     size_t n = node_at(a, cx, cy, cz, a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
     a.fl_cur[n] = FL_FLUID;
     a.key_cur[n] = 0xffffffffu;
@@ -231,7 +235,7 @@ __device__ __forceinline__ void k_unstamp_body(const PartArgs& a, int p) {
 __device__ __forceinline__ void k_stamp_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
-  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
     size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
     atomicMin(&a.key_cur[n], ((uint32_t)p << 3) | (uint32_t)a.tmpl[4 * t + 3]);
   }
@@ -240,12 +244,12 @@ __device__ __forceinline__ void k_stamp_body(const PartArgs& a, int p) {
 __device__ __forceinline__ void k_resolve_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
-  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
     size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
     uint32_t k = a.key_cur[n];
     if ((k >> 3) == (uint32_t)p) a.fl_cur[n] = (uint8_t)(k & 7u);
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && blockIdx.y == 0) {
     a.stamp_cur[3 * p] = c[0];
     a.stamp_cur[3 * p + 1] = c[1];
     a.stamp_cur[3 * p + 2] = c[2];
@@ -314,7 +318,7 @@ __device__ __forceinline__ void k_fill_claim_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
-  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.nball; t += TPB * gridDim.y) {
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2]);
     if (fill_candidate(a, n)) atomicMax(&a.claim[n], p);
   }
@@ -394,7 +398,7 @@ __device__ __forceinline__ void k_fill_release_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
-  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.nball; t += TPB * gridDim.y) {
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2]);
     a.claim[n] = -1;
   }
@@ -405,7 +409,7 @@ template <class S>
 __device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
-  for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntlist; t += TPB * gridDim.y) {
     int gx, gy, gz;
     size_t n = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
     if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) continue;
@@ -487,7 +491,7 @@ __device__ __forceinline__ void k_shell_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
   /* the particle's own nodes: MINDENS (every template node is non-fluid) */
-  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
     const size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
     if ((a.key_cur[n] >> 3) != (uint32_t)p) continue;
     a.rho_r[n] = 0.0f;
@@ -495,7 +499,7 @@ __device__ __forceinline__ void k_shell_body(const PartArgs& a, int p) {
     a.phi[n] = (0.0f - 0.0f) / (0.0f + 0.0f);
   }
   /* the ring around its surface: densities of the fluid nodes */
-  for (int t = threadIdx.x; t < a.nring; t += TPB) {
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.nring; t += TPB * gridDim.y) {
     const size_t n = node_at(a, c[0], c[1], c[2], a.ring[4 * t], a.ring[4 * t + 1], a.ring[4 * t + 2]);
     const uint8_t f = a.fl_cur[n];
     float rr, rb;
@@ -750,14 +754,60 @@ __device__ __forceinline__ void k_neigh_body(const PartArgs& a, int i) {
 struct PairParams {
   double hz_k, hz_rmin, hz_rcap, gmin;
   double rpar, lubfactor, rparcut, rparcap, mxrsqcut, visc;
+  double closure_r;   /* centre distance below which the closure may act */
   int hertz, lub;
 };
+
+/* learned near-contact correction of pair (i, j) (the closure block layout
+ * of LBM_CLOSURE_NFLOAT): adds the force and torque on i and returns true
+ * when the pair's features lie inside the training box */
+__device__ bool closure_pair(const PartArgs& a, int i, int j, double r, const double n[3], double radius,
+                             double f[6]) {
+  const float* cb = a.closure;
+  const float* mean = cb + LBM_MLP_NPARAM;
+  const float* scale = mean + 4;
+  const float* oscale = scale + 4;
+  const float* lo = oscale + 3;
+  const float* hi = lo + 4;
+  double oi[3] = {0, 0, 0}, oj[3] = {0, 0, 0};
+  if (a.lrotate) {
+    omega_lab(a.pos_cur, i, oi);
+    omega_lab(a.pos_cur, j, oj);
+  }
+  const double os[3] = {oi[0] + oj[0], oi[1] + oj[1], oi[2] + oj[2]};
+  double ocn[3];
+  cross3(os, n, ocn);
+  double u[3];
+  for (int d = 0; d < 3; ++d) u[d] = a.pos_cur[13 * i + 3 + d] - a.pos_cur[13 * j + 3 + d] + radius * ocn[d];
+  const double vn = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+  double t[3] = {u[0] - vn * n[0], u[1] - vn * n[1], u[2] - vn * n[2]};
+  const double vt = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  if (vt > 1e-300)
+    for (int d = 0; d < 3; ++d) t[d] /= vt;
+  else
+    t[0] = t[1] = t[2] = 0.0;
+  const float feat[4] = {(float)((r - 2.0 * radius) / radius), (float)vn, (float)vt, (float)radius};
+  for (int k = 0; k < 4; ++k)
+    if (!(feat[k] >= lo[k] && feat[k] <= hi[k])) return false;
+  float in[4], out[3];
+  for (int k = 0; k < 4; ++k) in[k] = (feat[k] - mean[k]) / scale[k];
+  lbm_mlp_eval(cb, in, out);
+  const double Fn = (double)(out[0] * oscale[0]), Ft = (double)(out[1] * oscale[1]), Tq = (double)(out[2] * oscale[2]);
+  double nt[3];
+  cross3(n, t, nt);
+  for (int d = 0; d < 3; ++d) {
+    f[d] -= Fn * n[d] + Ft * t[d];
+    f[3 + d] += Tq * nt[d];
+  }
+  return true;
+}
 
 __device__ __forceinline__ void k_pairs_body(const PartArgs& a, int i, const PairParams& q) {
   /* one workgroup per particle i, its threads over the partners j; FP64
      partial forces reduced in a fixed order */
   double f[6] = {0, 0, 0, 0, 0, 0};
   const double xi = a.pos_cur[13 * i], yi = a.pos_cur[13 * i + 1], zi = a.pos_cur[13 * i + 2];
+  const double cutsq = a.closure ? fmax(q.mxrsqcut, q.closure_r * q.closure_r) : q.mxrsqcut;
   for (int j = threadIdx.x; j < a.np; j += TPB) {
     if (j == i) continue;
     /* addHertzian, PART:1385-1503, with the pair vector from i to j */
@@ -766,13 +816,20 @@ __device__ __forceinline__ void k_pairs_body(const PartArgs& a, int i, const Pai
     ydf -= (double)a.ny * (double)nint(ydf / (double)a.ny);
     zdf -= (double)a.nz * (double)nint(zdf / (double)a.nz);
     const double rsq = xdf * xdf + ydf * ydf + zdf * zdf;
-    if (rsq > q.mxrsqcut) continue;
+    if (rsq > cutsq) continue;
     const double rrr = sqrt(rsq);
     if (q.hertz && rrr <= q.hz_rmin) {
       const double g = rrr <= q.hz_rcap ? q.gmin : 2.5 * q.hz_k * pow(q.hz_rmin - rrr, 1.5);
       f[0] -= g * xdf / rrr; f[1] -= g * ydf / rrr; f[2] -= g * zdf / rrr;
     }
-    if (q.lub && rrr <= q.rparcut) {
+    /* near contact: the learned correction where it was trained, the
+       reference's lubrication term elsewhere */
+    bool learned = false;
+    if (a.closure && rrr < q.closure_r) {
+      const double n[3] = {xdf / rrr, ydf / rrr, zdf / rrr};
+      learned = closure_pair(a, i, j, rrr, n, q.rpar * 0.5, f);
+    }
+    if (!learned && q.lub && rrr <= q.rparcut) {
       const double ux = xdf / rrr, uy = ydf / rrr, uz = zdf / rrr;
       const double rr = rrr <= q.rparcap ? q.rparcap : rrr;
       const double dotuv = ux * (a.pos_cur[13 * i + 3] - a.pos_cur[13 * j + 3]) +
@@ -783,8 +840,11 @@ __device__ __forceinline__ void k_pairs_body(const PartArgs& a, int i, const Pai
     }
   }
   block_reduce6(f);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     for (int d = 0; d < 3; ++d) a.myf2[3 * i + d] += f[d];
+    if (a.lrotate && a.closure)
+      for (int d = 0; d < 3; ++d) a.myt2[3 * i + d] += f[3 + d];
+  }
 }
 
 /* ---------------- rigid-body update --------------------------------------
@@ -999,6 +1059,7 @@ PartArgs make_pargs(lbm_ctx* c) {
   a.radius = c->cfg.radius;
   a.lrotate = c->cfg.lrotate;
   a.wetting = c->cfg.wetting;
+  a.closure = c->d_closure;
   a.rho_from_pops = c->variant == LBM_KERNEL_FUSED ? 1 : 0;
   a.lag_from_pops = (c->variant == LBM_KERNEL_FUSED && c->step > 1) ? 1 : 0;
   a.status = c->d_status;
@@ -1030,15 +1091,24 @@ int lbm_particles_upload_belt2() {
   return hipMemcpyToSymbol(HIP_SYMBOL(C_BELT2), t, sizeof t) == hipSuccess ? 0 : LBM_ERR_HIP;
 }
 
+/* workgroups per particle for the chunked kernels: about 4 entries of the
+ * largest offset list (template or ball) per lane */
+unsigned lbm_particles_chunks(int ntmpl, int nball) {
+  const int n = ntmpl > nball ? ntmpl : nball;
+  const int c = (n + 4 * TPB - 1) / (4 * TPB);
+  return (unsigned)(c < 1 ? 1 : c);
+}
+
 int lbm_particles_flags(lbm_ctx* c, int level) {
   PartArgs a = make_pargs(c);
   a.pos_cur = c->d_pos[level];
   a.fl_cur = c->flags[level];
   a.key_cur = c->d_key[level];
   a.stamp_cur = c->d_stamp[level];
-  hipLaunchKernelGGL(k_unstamp, dim3(c->np), dim3(TPB), 0, c->stream, a);
-  hipLaunchKernelGGL(k_stamp, dim3(c->np), dim3(TPB), 0, c->stream, a);
-  hipLaunchKernelGGL(k_resolve, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
+  hipLaunchKernelGGL(k_unstamp, g, dim3(TPB), 0, c->stream, a);
+  hipLaunchKernelGGL(k_stamp, g, dim3(TPB), 0, c->stream, a);
+  hipLaunchKernelGGL(k_resolve, g, dim3(TPB), 0, c->stream, a);
   return last_err();
 }
 
@@ -1050,22 +1120,25 @@ int lbm_particles_erase(lbm_ctx* c) {
 
 int lbm_particles_fill(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
-  hipLaunchKernelGGL(k_fill_claim, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
+  hipLaunchKernelGGL(k_fill_claim, g, dim3(TPB), 0, c->stream, a);
   LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_fill_apply<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
-  hipLaunchKernelGGL(k_fill_release, dim3(c->np), dim3(TPB), 0, c->stream, a);
+  hipLaunchKernelGGL(k_fill_release, g, dim3(TPB), 0, c->stream, a);
   return last_err();
 }
 
 int lbm_particles_wetting(lbm_ctx* c) {
   if (c->cfg.wetting == LBM_WET_NONE) return 0;
   PartArgs a = make_pargs(c);
-  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_wetting<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_wetting<S>, g, dim3(TPB), 0, c->stream, a))
   return last_err();
 }
 
 int lbm_particles_shell(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
-  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shell<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shell<S>, g, dim3(TPB), 0, c->stream, a))
   return last_err();
 }
 
@@ -1094,6 +1167,7 @@ PairParams make_pair_params(const lbm_ctx* c) {
   pp.visc = ((double)g.tau_r - 0.5) / 3.0;
   pp.hertz = g.hertz;
   pp.lub = g.lubrication;
+  pp.closure_r = pp.rpar + (double)c->closure_gap_max;
   return pp;
 }
 
@@ -1129,12 +1203,14 @@ struct lbm_part_batch {
   IntParams* d_ip;
   int2* d_items;           /* (box, particle) */
   int nitems;
+  unsigned chunks;         /* workgroups per particle of the chunked kernels */
 };
 
 lbm_part_batch* lbm_part_batch_create(lbm_ctx* const* boxes, int nbox, int* rc) {
   *rc = 0;
   lbm_part_batch* b = new lbm_part_batch();
   b->nbox = nbox;
+  b->chunks = 1;
   b->storage = boxes[0]->storage;
   std::vector<PartArgs> args[2];
   std::vector<PairParams> pp(nbox);
@@ -1152,6 +1228,10 @@ lbm_part_batch* lbm_part_batch_create(lbm_ctx* const* boxes, int nbox, int* rc) 
     pp[i] = make_pair_params(c);
     ip[i] = make_int_params(c);
     for (int q = 0; q < c->np; ++q) items.push_back(make_int2(i, q));
+    if (c->np > 0) {
+      const unsigned ch = lbm_particles_chunks(c->ntmpl, c->nball);
+      if (ch > b->chunks) b->chunks = ch;
+    }
   }
   b->nitems = (int)items.size();
   b->d_args[0] = b->d_args[1] = nullptr;
@@ -1187,17 +1267,17 @@ int lbm_part_batch_pre(lbm_part_batch* b, int parity, long step, hipStream_t st)
   BatchStep bs;
   bs.step = (int)step;
   bs.lag_from_pops = step > 1 ? 1 : 0;
-  const dim3 g((unsigned)b->nitems), t(TPB);
-  hipLaunchKernelGGL(k_unstamp_batch, g, t, 0, st, A, b->d_items, bs);
-  hipLaunchKernelGGL(k_stamp_batch, g, t, 0, st, A, b->d_items, bs);
-  hipLaunchKernelGGL(k_resolve_batch, g, t, 0, st, A, b->d_items, bs);
+  const dim3 g((unsigned)b->nitems), gc((unsigned)b->nitems, b->chunks), t(TPB);
+  hipLaunchKernelGGL(k_unstamp_batch, gc, t, 0, st, A, b->d_items, bs);
+  hipLaunchKernelGGL(k_stamp_batch, gc, t, 0, st, A, b->d_items, bs);
+  hipLaunchKernelGGL(k_resolve_batch, gc, t, 0, st, A, b->d_items, bs);
   LBM_STORAGE_SWITCH(b->storage,
     hipLaunchKernelGGL(k_erase_batch<S>, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_shell_batch<S>, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_fill_claim_batch, g, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_shell_batch<S>, gc, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_fill_claim_batch, gc, t, 0, st, A, b->d_items, bs);
     hipLaunchKernelGGL(k_fill_apply_batch<S>, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_fill_release_batch, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_wetting_batch<S>, g, t, 0, st, A, b->d_items, bs))
+    hipLaunchKernelGGL(k_fill_release_batch, gc, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_wetting_batch<S>, gc, t, 0, st, A, b->d_items, bs))
   return last_err();
 }
 

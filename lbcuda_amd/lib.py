@@ -64,6 +64,10 @@ def lib():
     L.lbm_tile_shape.argtypes = [i32, P(i32), P(i32), P(i32)]
     L.lbm_autotune.argtypes = [vp, i32, P(i32), P(i32), P(ctypes.c_double)]
     L.lbm_copy_bandwidth.argtypes = [ctypes.c_int64, i32, P(ctypes.c_double)]
+    L.lbm_mlp_forward.argtypes = [vp, vp, vp, i32]
+    L.lbm_mlp_loss_grad.argtypes = [vp, vp, vp, i32, P(ctypes.c_double), vp]
+    L.lbm_mlp_train.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, P(ctypes.c_double)]
+    L.lbm_set_closure.argtypes = [vp, vp]
     L.lbm_ensemble_create.argtypes = [i32, P(LbmConfig), P(vp)]
     L.lbm_ensemble_destroy.argtypes = [vp]
     L.lbm_ensemble_box.argtypes = [vp, i32, P(vp)]
@@ -79,7 +83,8 @@ def lib():
                  "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_set_kernel", "lbm_get_kernel",
                  "lbm_tile_shape", "lbm_autotune", "lbm_copy_bandwidth", "lbm_ensemble_create",
                  "lbm_ensemble_destroy", "lbm_ensemble_box", "lbm_ensemble_step", "lbm_ensemble_set_tile",
-                 "lbm_ensemble_autotune"):
+                 "lbm_ensemble_autotune", "lbm_mlp_forward", "lbm_mlp_loss_grad", "lbm_mlp_train",
+                 "lbm_set_closure"):
         getattr(L, name).restype = ctypes.c_int
     _LIB = L
     return L
@@ -91,7 +96,8 @@ EXPORTED = ["lbm_config_default", "lbm_create", "lbm_destroy", "lbm_init_fields"
             "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version", "lbm_fill_random_rho",
             "lbm_set_kernel", "lbm_get_kernel", "lbm_num_tiles", "lbm_tile_shape", "lbm_autotune",
             "lbm_copy_bandwidth", "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_box",
-            "lbm_ensemble_step", "lbm_ensemble_set_tile", "lbm_ensemble_autotune"]
+            "lbm_ensemble_step", "lbm_ensemble_set_tile", "lbm_ensemble_autotune", "lbm_mlp_forward",
+            "lbm_mlp_loss_grad", "lbm_mlp_train", "lbm_set_closure"]
 
 
 def _f32(a):
@@ -229,6 +235,16 @@ class Box:
                     "lbm_autotune")
         return t.value, z.value, ms.value
 
+    def set_closure(self, block):
+        """Enable the learned near-contact closure (block of
+        MLP_CLOSURE_NFLOAT floats, see lbcuda_amd.closure) or disable it
+        (None)."""
+        if block is None:
+            self._check(lib().lbm_set_closure(self.h, None), "lbm_set_closure")
+            return
+        b = np.ascontiguousarray(block, dtype=np.float32)
+        self._check(lib().lbm_set_closure(self.h, _ptr(b)), "lbm_set_closure")
+
     def kernel_info(self):
         buf = ctypes.create_string_buffer(128)
         b = ctypes.c_double()
@@ -319,6 +335,46 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
     if rc:
         raise LbmError(rc, "lbm_copy_bandwidth")
     return g.value
+
+
+MLP_NPARAM = 4675
+MLP_CLOSURE_NFLOAT = MLP_NPARAM + 20
+
+
+def mlp_forward(params, x):
+    """GPU batch inference of the closure network: x (n,4) -> y (n,3)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((xx.shape[0], 3), dtype=np.float32)
+    rc = lib().lbm_mlp_forward(_ptr(p), _ptr(xx), _ptr(y), xx.shape[0])
+    if rc:
+        raise LbmError(rc, "lbm_mlp_forward")
+    return y
+
+
+def mlp_loss_grad(params, x, t):
+    """Mean squared error and its gradient, computed by the GPU kernels."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    loss = ctypes.c_double()
+    rc = lib().lbm_mlp_loss_grad(_ptr(p), _ptr(xx), _ptr(tt), xx.shape[0], ctypes.byref(loss), _ptr(g))
+    if rc:
+        raise LbmError(rc, "lbm_mlp_loss_grad")
+    return loss.value, g
+
+
+def mlp_train(params, x, t, epochs, batch=256, lr=1e-3, seed=0):
+    """Adam training on the GPU; returns (trained params, final loss)."""
+    p = np.array(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    loss = ctypes.c_double()
+    rc = lib().lbm_mlp_train(_ptr(p), _ptr(xx), _ptr(tt), xx.shape[0], epochs, batch, lr, seed, ctypes.byref(loss))
+    if rc:
+        raise LbmError(rc, "lbm_mlp_train")
+    return p, loss.value
 
 
 def version():
