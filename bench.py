@@ -88,6 +88,33 @@ def run_gpu(args, storage, rr, rb, pos):
                 achieved_gbs=achieved, mass=sums[0:2].tolist())
 
 
+def run_ensemble(args):
+    """64 independent CG boxes of uneven size (edges 48..96, 2..16 spheres of
+    radius 4..8 each), one persistent fluid launch per step."""
+    from lbcuda_amd.lib import Ensemble
+    from lbcuda_amd.abi import INIT_RHO
+    from lbcuda_amd import workloads as wl
+    cfgs, inputs = wl.ensemble_workload(64, storage=0)
+    ens = Ensemble(cfgs)
+    for b, (rr, rb, pos) in zip(ens.boxes, inputs):
+        b.set_particles(pos)
+        b.init_fields(INIT_RHO, rho_r=rr, rho_b=rb)
+    tile, zc, _ = ens.autotune(3)
+    ens.step(5)
+    steps = max(10, args.steps // 4)
+    t0 = time.perf_counter()
+    rc = ens.step(steps)
+    dt = time.perf_counter() - t0
+    ens.close()
+    if rc:
+        raise RuntimeError(f"ensemble step returned {rc}")
+    nodes = sum(c.nx * c.ny * c.nz for c in cfgs)
+    log(f"ensemble: 64 boxes, {nodes} nodes, {dt / steps * 1e3:.3f} ms/step")
+    return {"boxes": 64, "nodes": nodes, "particles": sum(c.n_particles for c in cfgs), "steps": steps,
+            "ms_per_step": round(dt / steps * 1e3, 4), "mlups": round(nodes * steps / dt / 1e6, 1),
+            "tile": tile, "zchunk": zc}
+
+
 def cpu_baseline(args, rr, rb, pos):
     from oracle.pyoracle import OracleBox, lib as orc_lib
     from lbcuda_amd.abi import INIT_RHO
@@ -122,6 +149,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one box on one GPU; multi-GPU runs are not part of it "
@@ -137,6 +165,7 @@ def main():
     log(f"device copy kernel: {copy_gbs:.0f} GB/s (read + write)")
     r32 = run_gpu(args, 0, rr, rb, pos)
     r16 = None if args.no_fp16 else run_gpu(args, 1, rr, rb, pos)
+    ens = None if args.no_ensemble else run_ensemble(args)
     cpu = None if args.no_cpu else cpu_baseline(args, rr, rb, pos)
 
     traffic = traffic_from_profiles("f32")
@@ -165,6 +194,7 @@ def main():
         "mlups_ref_style": round(r32["mlups"] * ((args.n + 2) / args.n) ** 3, 1),
         "copy_kernel_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     if r16:
         out["fp16_store"] = {"mlups": round(r16["mlups"], 1), "ms_per_step": round(r16["ms_per_step"], 4),

@@ -9,17 +9,17 @@ that the Reynolds number and the path in units of R are the same and the
 fine run takes 16x the steps).  Runs are head-on and oblique (relative
 velocity at an angle to the initial line of centres, so the pair slides
 past at closest approach); the coarse gap goes from about 3 down to 0.1
-lattice units.  Pair forces (Hertz, lubrication) are off, so the measured
+lattice units after a warm-up path of 2.  Pair forces (Hertz, lubrication) are off, so the measured
 force is hydrodynamic only.
 
 Along each run the force on sphere 1 is split into a normal part F_n
 (repulsive > 0), a tangential part F_t (opposing the relative sliding > 0)
 and the torque T about n x t, and made dimensionless with the Stokes scales
-6 pi mu R |u| (forces) and 8 pi mu R^2 |u| (torque).  Both runs are averaged
-in bins of the path length; the training target at a coarse bin is the
-fine coefficient at the same path position minus the coarse one, in coarse
-force units: what the coarse lattice misses.  Features (gap/R, v_n, v_t, R)
-are those lbm_set_closure documents.
+6 pi mu R |u| (forces) and 8 pi mu R^2 |u| (torque).  Both runs are reduced
+to medians in bins of the coarse-unit gap (0.1 wide, 0.35 to 3); the
+training target of a bin is the fine coefficient minus the coarse one, in
+coarse force units: what the coarse lattice misses.  Features (gap/R, v_n,
+v_t, R) are those lbm_set_closure documents.
 
 The network (4 -> 64 -> 64 -> 3) is trained with the library's own GPU
 kernels (lbm_mlp_train); lbm_set_closure then applies it in the pair-force
@@ -27,6 +27,7 @@ kernel for pairs with gap < gap_max inside the feature box of the data, the
 reference's lubrication term elsewhere (kernels_fluid_PART.CUF:1463-1501).
 """
 import math
+import os
 
 import numpy as np
 
@@ -62,11 +63,12 @@ def _path(R, angle_deg):
 SPEED = 4e-3                    # relative speed at the coarse resolution (Re = U R / nu <= 0.2)
 
 
-def run_pair(R, angle_deg, scale, speed=SPEED, sample_every=None, storage=0):
+def run_pair(R, angle_deg, scale, speed=SPEED, sample_every=None, storage=0, closure=None):
     """One pair-approach run at `scale` x resolution (radius R*scale, box and
     path scaled, relative speed speed/scale).  Returns per-sample arrays:
     path s (coarse units), gap/R, v_n/|u|, v_t/|u| and the dimensionless
-    C_n, C_t, C_T."""
+    C_n, C_t, C_T of the total force (and Bn, Bt, BT of the link part
+    alone).  `closure`: a closure block applied in the pair-force kernel."""
     Rs = R * scale
     us = speed / scale
     r0, e_u, total = _path(R, angle_deg)
@@ -79,9 +81,11 @@ def run_pair(R, angle_deg, scale, speed=SPEED, sample_every=None, storage=0):
     dims = [d + (d % 2) for d in dims]
     cfg = default_config(dims[0], sigma=0.0, beta=0.0, storage=storage, with_particles=1, n_particles=2,
                          radius=float(Rs), lrotate=1, rmass=0.0, max_vel=1.0, wetting=WET_NEUTRAL, hertz=0,
-                         lubrication=0)
+                         lubrication=0, closure=int(closure is not None))
     cfg.nx, cfg.ny, cfg.nz = dims
     box = L.Box(cfg)
+    if closure is not None:
+        box.set_closure(closure)
     n3 = np.array(dims, dtype=np.float64)
     origin = n3 / 2.0 + 0.5 - 0.5 * (lo + hi) * scale
     pos = np.zeros((13, 2), order="F")
@@ -130,47 +134,63 @@ def run_pair(R, angle_deg, scale, speed=SPEED, sample_every=None, storage=0):
     return {k: np.array(v) for k, v in out.items()}
 
 
-def _bin(run, edges):
-    idx = np.digitize(run["s"], edges) - 1
-    cols = {}
-    for k in ("gap", "vn", "vt", "Cn", "Ct", "CT"):
-        v = np.full(len(edges) - 1, np.nan)
-        for b in range(len(edges) - 1):
-            m = idx == b
-            if m.any():
-                v[b] = run[k][m].mean()
-        cols[k] = v
-    return cols
+GAP_BIN = 0.1        # coarse lattice units
+GAP_TRAIN_MIN = 0.35  # below this the 4x run itself has < 1.4 nodes in the gap
+OUTLIER = 20.0        # |coefficient| above this: a fill/erase or shared-node spike
 
 
-def dataset(radii=(4.0, 6.0, 8.0), angles=(0.0, 40.0), speed=SPEED, nbins=40, storage=0, log=None):
-    """Coarse and fine pair runs, binned along the path; one row per bin
-    with gap/R <= GAP_START/R: features (gap/R, v_n, v_t, R) in coarse units,
-    coarse and fine coefficients, and the run id."""
+def gap_bins(run, R):
+    """Bin index (GAP_BIN wide from GAP_TRAIN_MIN, coarse units) of every
+    sample of a run after the warm-up path; -1 outside the binned range and
+    for non-finite samples (a node uncovered with no fluid node within two
+    belts gets zero density, as in the reference, and poisons the run from
+    there on; at 4x resolution that happens below a coarse gap of 0.1)."""
+    g = run["gap"] * R
+    ok = np.isfinite(g) & (run["s"] > WARM)
+    b = np.floor((np.where(ok, g, -1.0) - GAP_TRAIN_MIN) / GAP_BIN).astype(int)
+    nb = int(round((GAP_START - GAP_TRAIN_MIN) / GAP_BIN))
+    return np.where(ok & (b >= 0) & (b < nb), b, -1)
+
+
+def bin_runs(c, f, R, speed, run_id):
+    """Median coefficients of a coarse/fine run pair in bins of the coarse
+    gap (GAP_BIN wide, from GAP_TRAIN_MIN to GAP_START), after the warm-up
+    path.  Medians, not means: fill/erase events put impulses of tens of
+    Stokes units on single steps.  Bins with a median beyond OUTLIER (an
+    artefact of surface nodes shared by the two spheres that lasts for the
+    whole bin) are dropped.  Rows follow COLS."""
     rows = []
-    raw = {}
+    bc, bf = gap_bins(c, R), gap_bins(f, R)
+    keys = ("gap", "vn", "vt", "Cn", "Ct", "CT")
+    for b in range(int(round((GAP_START - GAP_TRAIN_MIN) / GAP_BIN))):
+        mc, mf = bc == b, bf == b
+        if not (mc.any() and mf.any()):
+            continue
+        vc = {k: float(np.median(c[k][mc])) for k in keys}
+        vf = {k: float(np.median(f[k][mf])) for k in keys}
+        coef = [vc["Cn"], vc["Ct"], vc["CT"], vf["Cn"], vf["Ct"], vf["CT"]]
+        if max(abs(x) for x in coef) > OUTLIER:
+            continue
+        rows.append([vc["gap"], vc["vn"] * speed, vc["vt"] * speed, R] + coef + [run_id, b])
+    return rows
+
+
+def dataset(radii=(4.0, 6.0, 8.0), angles=(0.0, 40.0), speed=SPEED, storage=0, log=None):
+    """Coarse and fine pair runs for every radius and angle, binned by
+    bin_runs; returns the rows (columns COLS)."""
+    rows = []
     for ri, R in enumerate(radii):
         for ai, ang in enumerate(angles):
             if log:
                 log(f"pair runs R={R} angle={ang}")
             c = run_pair(R, ang, 1, speed, storage=storage)
             f = run_pair(R, ang, FINE, speed, storage=storage)
-            for tag, r in (("c", c), ("f", f)):
-                for k, v in r.items():
-                    raw[f"{tag}_{ri}_{ai}_{k}"] = v
-            edges = np.linspace(WARM, c["s"].max(), nbins + 1)
-            bc, bf = _bin(c, edges), _bin(f, edges)
-            for b in range(nbins):
-                vals = [bc[k][b] for k in bc] + [bf[k][b] for k in bf]
-                if np.any(np.isnan(vals)):
-                    continue
-                rows.append([bc["gap"][b], bc["vn"][b] * speed, bc["vt"][b] * speed, R,
-                             bc["Cn"][b], bc["Ct"][b], bc["CT"][b], bf["Cn"][b], bf["Ct"][b], bf["CT"][b],
-                             ri * len(angles) + ai])
-    return np.array(rows), raw
+            rows += bin_runs(c, f, R, speed, ri * len(angles) + ai)
+    return np.array(rows)
 
 
-COLS = ["gap_R", "vn", "vt", "R", "Cn_c", "Ct_c", "CT_c", "Cn_f", "Ct_f", "CT_f", "run"]
+COLS = ["gap_R", "vn", "vt", "R", "Cn_c", "Ct_c", "CT_c", "Cn_f", "Ct_f", "CT_f", "run", "bin"]
+RUN_KEYS = ("s", "gap", "vn", "vt", "Cn", "Ct", "CT")
 
 
 def targets(data):
@@ -197,6 +217,25 @@ def lubrication_normal(data, lub_k=1.0, lub_rmin=1.0, lub_rcap=0.1):
     return np.where(r <= rpar + lub_rmin, f, 0.0)
 
 
+DATA = os.path.join(os.path.dirname(__file__), "data", "closure_pairs.npz")
+
+
+def load(path=DATA):
+    """The committed pair-approach dataset (rows of COLS), generated on an
+    MI355X by `python -m lbcuda_amd.closure generate`."""
+    return np.load(path)["data"]
+
+
+def split(data, hold_every=3):
+    """Training rows and held-out rows (every hold_every-th gap bin of each
+    run)."""
+    keep = np.ones(len(data), bool)
+    for run in np.unique(data[:, 10]):
+        idx = np.where(data[:, 10] == run)[0]
+        keep[idx[1::hold_every]] = False
+    return data[keep], data[~keep]
+
+
 def normalisation(x, y):
     mean = x.mean(0)
     scale = x.std(0)
@@ -220,7 +259,7 @@ def glorot(seed=0):
     return np.concatenate(parts).astype(np.float32)
 
 
-def fit(data, epochs=2000, batch=64, lr=2e-3, seed=1, gap_max=1.0):
+def fit(data, epochs=3000, batch=128, lr=2e-3, seed=1, gap_max=GAP_START):
     """Train the closure network with the GPU kernels.  Returns (block,
     final normalised loss)."""
     x = data[:, 0:4]
@@ -233,6 +272,21 @@ def fit(data, epochs=2000, batch=64, lr=2e-3, seed=1, gap_max=1.0):
     block = np.concatenate([p, mean, scale, oscale, lo, hi, [gap_max]]).astype(np.float32)
     assert block.size == L.MLP_CLOSURE_NFLOAT
     return block, loss
+
+
+def evaluate(block, data):
+    """Error of the coarse force with and without corrections against the
+    4x-resolved force on the data rows, per output (normal force, tangential
+    force, torque), as RMS in coarse force units: no correction, the
+    reference's lubrication term (normal only) and the learned closure."""
+    y = targets(data)
+    pred = predict(block, data)
+    lub = lubrication_normal(data)
+    rms = lambda e: np.sqrt(np.mean(e ** 2, axis=0))
+    none = rms(y)
+    lubr = rms(y - np.stack([lub, np.zeros_like(lub), np.zeros_like(lub)], 1))
+    learned = rms(y - pred)
+    return {"none": none, "lubrication": lubr, "learned": learned}
 
 
 def predict(block, data):
@@ -260,10 +314,9 @@ def main(argv=None):
     t0 = time.time()
     radii = tuple(float(x) for x in a.radii.split(","))
     angles = tuple(float(x) for x in a.angles.split(","))
-    d, raw = dataset(radii, angles, log=lambda m: print(f"[closure] {time.time() - t0:7.1f}s {m}", file=sys.stderr,
-                                                        flush=True))
+    d = dataset(radii, angles, log=lambda m: print(f"[closure] {time.time() - t0:7.1f}s {m}", file=sys.stderr,
+                                                  flush=True))
     np.savez(a.out, data=d, cols=np.array(COLS), speed=SPEED, radii=np.array(radii), angles=np.array(angles))
-    np.savez_compressed(a.out.replace(".npz", "_raw.npz"), **raw)
     print(f"[closure] {d.shape[0]} rows -> {a.out} in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
 

@@ -131,3 +131,48 @@ def test_pair_run_smoke():
     # an approaching pair is pushed apart while a fluid layer separates it
     far = c["gap"] > 0.5
     assert c["Bn"][far].mean() > 0 and f["Bn"][f["gap"] > 0.5].mean() > 0
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """The closure trained with lbm_mlp_train on the committed pair-approach
+    data, every third gap bin of each run held out."""
+    train, held = C.split(C.load())
+    block, loss = C.fit(train)
+    print(f"closure training loss {loss:.3e} on {len(train)} bins")
+    return block, train, held
+
+
+def test_learned_closure_gain_on_held_out_bins(trained):
+    """On gap bins it has not seen, coarse force + learned correction is
+    closer to the 4x-resolved force than the coarse force alone and than the
+    coarse force + the reference's lubrication term."""
+    block, train, held = trained
+    e = C.evaluate(block, held)
+    tot = {k: float(np.sqrt((v ** 2).sum())) for k, v in e.items()}
+    print({k: np.round(v, 4).tolist() for k, v in e.items()}, tot)
+    assert tot["learned"] < 0.7 * tot["none"]
+    assert e["learned"][0] < 0.7 * e["lubrication"][0]
+    # normal force and torque each improve
+    assert e["learned"][0] < e["none"][0] and e["learned"][2] < e["none"][2]
+
+
+@pytest.mark.parametrize("angle", [0.0, 40.0])
+def test_learned_closure_in_solver(trained, angle):
+    """The trained block set on a coarse box (R = 6): the pair-force kernel
+    evaluates the network on each step's features and adds it to the
+    hydrodynamic force.  Per held-out gap bin, the medians of the total
+    force coefficients move towards the 4x-resolved ones."""
+    block, train, held = trained
+    R = 6.0
+    run_id = 2 + int(angle > 0)
+    rows = held[held[:, 10] == run_id]
+    assert len(rows) >= 5
+    r = C.run_pair(R, angle, 1, closure=block)
+    bins = C.gap_bins(r, R)
+    got = np.array([[np.median(r[k][bins == int(b)]) for k in ("Cn", "Ct", "CT")] for b in rows[:, 11]])
+    fine, coarse = rows[:, 7:10], rows[:, 4:7]
+    err_on = np.sqrt(((got - fine) ** 2).sum())
+    err_off = np.sqrt(((coarse - fine) ** 2).sum())
+    print(f"angle {angle}: |C - C_fine| without closure {err_off:.4f}, with {err_on:.4f}")
+    assert err_on < 0.8 * err_off
