@@ -89,6 +89,67 @@ template <> struct Store<__hip_bfloat16> {
   __device__ static void st(__hip_bfloat16* p, size_t i, float v, float s) { p[i] = __float2bfloat16(v - s); }
 };
 
+/* Population layout.  Each time level holds, for every direction l, a plane of
+ * N colour pairs (f_R, f_B) of node n, x fastest: the pair (l, n) is element
+ * l N + n of an array of 2-value pairs (8 bytes for float, 4 bytes for the
+ * 16-bit types).  One vector access moves both colours of a node and
+ * direction, and a gather of whole nodes touches 19 cache lines per row, not
+ * 38.  Pair<S>::ld/st convert one pair with the shift s = w_l rho0 of the
+ * direction, element by element exactly as Store<S> does. */
+template <class S> struct Pair;
+template <> struct Pair<float> {
+  __device__ static void ld(const float* p, size_t i, float, float& r, float& b) {
+    const float2 v = reinterpret_cast<const float2*>(p)[i];
+    r = v.x;
+    b = v.y;
+  }
+  __device__ static void st(float* p, size_t i, float r, float b, float) {
+    reinterpret_cast<float2*>(p)[i] = make_float2(r, b);
+  }
+};
+template <> struct Pair<__half> {
+  __device__ static void ld(const __half* p, size_t i, float s, float& r, float& b) {
+    const __half2 v = reinterpret_cast<const __half2*>(p)[i];
+    r = __low2float(v) + s;
+    b = __high2float(v) + s;
+  }
+  __device__ static void st(__half* p, size_t i, float r, float b, float s) {
+    reinterpret_cast<__half2*>(p)[i] = __halves2half2(__float2half_rn(r - s), __float2half_rn(b - s));
+  }
+};
+template <> struct Pair<__hip_bfloat16> {
+  __device__ static void ld(const __hip_bfloat16* p, size_t i, float s, float& r, float& b) {
+    const uint32_t u = reinterpret_cast<const uint32_t*>(p)[i];
+    r = __uint_as_float(u << 16) + s;
+    b = __uint_as_float(u & 0xffff0000u) + s;
+  }
+  __device__ static void st(__hip_bfloat16* p, size_t i, float r, float b, float s) {
+    const uint32_t lo = __bfloat16_as_ushort(__float2bfloat16(r - s));
+    const uint32_t hi = __bfloat16_as_ushort(__float2bfloat16(b - s));
+    reinterpret_cast<uint32_t*>(p)[i] = lo | (hi << 16);
+  }
+};
+
+/* both colours of direction l at node n of a population level */
+template <class S>
+__device__ __forceinline__ void ld_pair(const void* pops, size_t N, int l, size_t n, float rho0, float& r, float& b) {
+  Pair<S>::ld((const S*)pops, (size_t)l * N + n, W(l) * rho0, r, b);
+}
+template <class S>
+__device__ __forceinline__ void st_pair(void* pops, size_t N, int l, size_t n, float r, float b, float rho0) {
+  Pair<S>::st((S*)pops, (size_t)l * N + n, r, b, W(l) * rho0);
+}
+/* raw copy of one pair (no conversion) */
+template <class S>
+__device__ __forceinline__ void cp_pair(void* dst, size_t di, const void* src, size_t si) {
+  if constexpr (sizeof(S) == 4)
+    reinterpret_cast<uint64_t*>(dst)[di] = reinterpret_cast<const uint64_t*>(src)[si];
+  else
+    reinterpret_cast<uint32_t*>(dst)[di] = reinterpret_cast<const uint32_t*>(src)[si];
+}
+/* one colour c of direction l at node n (host transfers) */
+__host__ __device__ constexpr size_t pop_index(size_t N, int c, int l, size_t n) { return 2 * ((size_t)l * N + n) + c; }
+
 /* device status word: first error wins */
 struct DevStatus {
   int code;

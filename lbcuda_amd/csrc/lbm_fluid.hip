@@ -1,10 +1,9 @@
 /*
  * lbm_fluid.hip -- fluid kernels of the colour-gradient step for gfx950.
  *
- * Layout in HBM: for each time level one buffer of 2 colours x 19 directions
- * x N nodes, direction-major SoA planes with x fastest, no halo
- * (reference: popsR_d/popsB_d(0:nx+1,0:ny+1,0:nz+1,0:18,1:2),
- * kernels_fluid.CUF:9-12).  Periodicity is done by index wrap inside the
+ * Layout in HBM: for each time level one buffer of 19 direction planes of N
+ * colour pairs (f_R, f_B), x fastest, no halo (reference:
+ * popsR_d/popsB_d(0:nx+1,0:ny+1,0:nz+1,0:18,1:2), kernels_fluid.CUF:9-12).  Periodicity is done by index wrap inside the
  * kernels; the reference's 24 halo-refresh/halo-stream launches per step
  * (main.CUF:577-1000, kernels_fluid_CG.CUF:1017-1520) have no counterpart.
  *
@@ -30,14 +29,6 @@ namespace {
 constexpr int BX = 64;   /* one wavefront per x-row segment */
 constexpr int BY = 4;
 
-template <class S>
-__device__ __forceinline__ float ldp(const S* p, size_t i, int l, float rho0) {
-  return Store<S>::ld(p, i, W(l) * rho0);
-}
-template <class S>
-__device__ __forceinline__ void stp(S* p, size_t i, int l, float v, float rho0) {
-  Store<S>::st(p, i, v, W(l) * rho0);
-}
 
 /* fcut, kernels_fluid_CG.CUF:127-138 */
 __device__ float fcut(float r, float r1, float r2) {
@@ -93,14 +84,12 @@ __global__ __launch_bounds__(256) void k_init_eq(FluidArgs a, int type, float vx
       w = vel[3 * n + 2];
     }
   }
-  S* dst = (S*)a.dst;          /* current level */
-  S* oth = (S*)a.src;          /* other level: cleared */
+  void* dst = a.dst;                /* current level */
+  void* oth = (void*)a.src;         /* other level: cleared */
 #pragma unroll
   for (int l = 0; l < 19; ++l) {
-    stp(dst, (size_t)l * a.N + n, l, equil(rr, u, v, w, l), a.rho0);
-    stp(dst, (size_t)(19 + l) * a.N + n, l, equil(rb, u, v, w, l), a.rho0);
-    stp(oth, (size_t)l * a.N + n, l, 0.0f, a.rho0);
-    stp(oth, (size_t)(19 + l) * a.N + n, l, 0.0f, a.rho0);
+    st_pair<S>(dst, a.N, l, n, equil(rr, u, v, w, l), equil(rb, u, v, w, l), a.rho0);
+    st_pair<S>(oth, a.N, l, n, 0.0f, 0.0f, a.rho0);
   }
   fl0[n] = FL_FLUID;
   fl1[n] = FL_FLUID;
@@ -115,15 +104,15 @@ template <class S>
 __global__ __launch_bounds__(256) void k_density(FluidArgs a) {
   size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.N) return;
-  const S* src = (const S*)a.src;
   float rr = 0.0f, rb = 0.0f;
   if (a.flags[n] == FL_FLUID) {
-    rr = ldp(src, n, 0, a.rho0);
-    rb = ldp(src, (size_t)19 * a.N + n, 0, a.rho0);
+    ld_pair<S>(a.src, a.N, 0, n, a.rho0, rr, rb);
 #pragma unroll
     for (int l = 1; l < 19; ++l) {
-      rr = rr + ldp(src, (size_t)l * a.N + n, l, a.rho0);
-      rb = rb + ldp(src, (size_t)(19 + l) * a.N + n, l, a.rho0);
+      float r, b;
+      ld_pair<S>(a.src, a.N, l, n, a.rho0, r, b);
+      rr = rr + r;
+      rb = rb + b;
     }
     if (rr < a.min_rho || rr > a.max_rho || rb < a.min_rho || rb > a.max_rho)
       raise_error(a.status, LBM_ERR_RHO_RANGE, a.step);
@@ -162,17 +151,14 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
                         (size_t)(z == nz - 1 ? 0 : z + 1) * sxy};
   const size_t n = (size_t)x + ys[1] + zs[1];
   auto nb = [&](int dx, int dy, int dz) -> size_t { return (size_t)xs[dx + 1] + ys[dy + 1] + zs[dz + 1]; };
-  const S* src = (const S*)a.src;
-  S* dst = (S*)a.dst;
+  const void* src = a.src;
+  void* dst = a.dst;
   const uint8_t f = a.flags[n];
 
   if (f == FL_FLUID) {
     float fr[19], fb[19];
 #pragma unroll
-    for (int l = 0; l < 19; ++l) {
-      fr[l] = ldp(src, (size_t)l * N + n, l, a.rho0);
-      fb[l] = ldp(src, (size_t)(19 + l) * N + n, l, a.rho0);
-    }
+    for (int l = 0; l < 19; ++l) ld_pair<S>(src, N, l, n, a.rho0, fr[l], fb[l]);
     const float rR = a.rho_r[n], rB = a.rho_b[n];
     const float rs = rR + rB;
     const float phf = a.phi[n];
@@ -227,14 +213,9 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
       }
     }
     /* rest population stays, the others move to x + e_l */
-    stp(dst, n, 0, fr[0], a.rho0);
-    stp(dst, (size_t)19 * N + n, 0, fb[0], a.rho0);
+    st_pair<S>(dst, N, 0, n, fr[0], fb[0], a.rho0);
 #pragma unroll
-    for (int l = 1; l < 19; ++l) {
-      const size_t t = nb(EX(l), EY(l), EZ(l));
-      stp(dst, (size_t)l * N + t, l, fr[l], a.rho0);
-      stp(dst, (size_t)(19 + l) * N + t, l, fb[l], a.rho0);
-    }
+    for (int l = 1; l < 19; ++l) st_pair<S>(dst, N, l, nb(EX(l), EY(l), EZ(l)), fr[l], fb[l], a.rho0);
     if (PART) {
       /* bounce-back copy: surface node s = x + e_l whose far side s + e_l is
          not fluid sends f_l(x) back to x in direction opp(l) */
@@ -243,14 +224,12 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
         if (a.flags[nb(EX(l), EY(l), EZ(l))] != FL_LIST) continue;
         const int x2 = wrapc(x + 2 * EX(l), nx), y2 = wrapc(y + 2 * EY(l), ny), z2 = wrapc(z + 2 * EZ(l), nz);
         if (a.flags[(size_t)x2 + (size_t)nx * (y2 + (size_t)ny * z2)] == FL_FLUID) continue;
-        stp(dst, (size_t)OPP(l) * N + n, OPP(l), fr[l], a.rho0);
-        stp(dst, (size_t)(19 + OPP(l)) * N + n, OPP(l), fb[l], a.rho0);
+        st_pair<S>(dst, N, OPP(l), n, fr[l], fb[l], a.rho0);
       }
     }
   } else {
     /* non-fluid: raw copies of the stored values (no collision, :689) */
-    dst[n] = src[n];
-    dst[(size_t)19 * N + n] = src[(size_t)19 * N + n];
+    cp_pair<S>(dst, n, src, n);
     if (f == FL_DEAD) return;
 #pragma unroll
     for (int l = 1; l < 19; ++l) {
@@ -258,8 +237,7 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
       if (PART && f == FL_LIST) {
         if (a.flags[t] == FL_FLUID && a.flags[nb(-EX(l), -EY(l), -EZ(l))] != FL_FLUID) continue;
       }
-      dst[(size_t)l * N + t] = src[(size_t)l * N + n];
-      dst[(size_t)(19 + l) * N + t] = src[(size_t)(19 + l) * N + n];
+      cp_pair<S>(dst, (size_t)l * N + t, src, (size_t)l * N + n);
     }
   }
 }
@@ -269,13 +247,9 @@ template <class S>
 __global__ __launch_bounds__(256) void k_moments(FluidArgs a, float* rr_o, float* rb_o, float* vel_o, float* ph_o) {
   size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.N) return;
-  const S* src = (const S*)a.src;
   float fr[19], fb[19];
 #pragma unroll
-  for (int l = 0; l < 19; ++l) {
-    fr[l] = ldp(src, (size_t)l * a.N + n, l, a.rho0);
-    fb[l] = ldp(src, (size_t)(19 + l) * a.N + n, l, a.rho0);
-  }
+  for (int l = 0; l < 19; ++l) ld_pair<S>(a.src, a.N, l, n, a.rho0, fr[l], fb[l]);
   float lr = fr[0], lb = fb[0];
 #pragma unroll
   for (int l = 1; l < 19; ++l) { lr = lr + fr[l]; lb = lb + fb[l]; }
@@ -304,13 +278,13 @@ template <class S>
 __global__ __launch_bounds__(256) void k_sums_partial(FluidArgs a, double* part) {
   __shared__ double sh[5][256];
   double acc[5] = {0, 0, 0, 0, 0};
-  const S* src = (const S*)a.src;
   for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += (size_t)gridDim.x * blockDim.x) {
     if (a.flags[n] != FL_FLUID) continue;
 #pragma unroll
     for (int l = 0; l < 19; ++l) {
-      double r = ldp(src, (size_t)l * a.N + n, l, a.rho0);
-      double b = ldp(src, (size_t)(19 + l) * a.N + n, l, a.rho0);
+      float rf, bf;
+      ld_pair<S>(a.src, a.N, l, n, a.rho0, rf, bf);
+      const double r = rf, b = bf;
       acc[0] += r;
       acc[1] += b;
       acc[2] += (r + b) * EX(l);
@@ -352,15 +326,15 @@ template <class S>
 __global__ __launch_bounds__(256) void k_rho_from_pops(FluidArgs a, const uint8_t* fl, float* rr_o, float* rb_o) {
   size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.N) return;
-  const S* src = (const S*)a.src;
   float rr, rb;
   if (fl[n] == FL_FLUID) {
-    rr = ldp(src, n, 0, a.rho0);
-    rb = ldp(src, (size_t)19 * a.N + n, 0, a.rho0);
+    ld_pair<S>(a.src, a.N, 0, n, a.rho0, rr, rb);
 #pragma unroll
     for (int l = 1; l < 19; ++l) {
-      rr = rr + ldp(src, (size_t)l * a.N + n, l, a.rho0);
-      rb = rb + ldp(src, (size_t)(19 + l) * a.N + n, l, a.rho0);
+      float r, b;
+      ld_pair<S>(a.src, a.N, l, n, a.rho0, r, b);
+      rr = rr + r;
+      rb = rb + b;
     }
     if (rr < 0.0f) rr = 0.0f;
     if (rb < 0.0f) rb = 0.0f;
@@ -377,7 +351,7 @@ __global__ __launch_bounds__(256) void k_unpack(FluidArgs a, int color, float* o
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N * 19) return;
   int l = (int)(i / a.N);
-  out[i] = ldp((const S*)a.src, (size_t)color * 19 * a.N + i, l, a.rho0);
+  out[i] = Store<S>::ld((const S*)a.src, 2 * i + (size_t)color, W(l) * a.rho0);
 }
 
 template <class S>
@@ -385,8 +359,7 @@ __global__ __launch_bounds__(256) void k_pack(FluidArgs a, const float* pr, cons
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N * 19) return;
   int l = (int)(i / a.N);
-  stp((S*)a.dst, i, l, pr[i], a.rho0);
-  stp((S*)a.dst, (size_t)19 * a.N + i, l, pb[i], a.rho0);
+  Pair<S>::st((S*)a.dst, i, pr[i], pb[i], W(l) * a.rho0);
 }
 
 /* plain device copy (read n float4, write n float4): the measured HBM

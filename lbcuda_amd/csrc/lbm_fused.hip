@@ -46,21 +46,25 @@ using namespace lbm;
 
 namespace {
 
-/* Population access with a uniform per-direction base pointer and a 32-bit
- * byte offset per lane (node index < 2^30), so loads and stores use the
- * SGPR-base + VGPR-offset address form instead of 64-bit VGPR addresses. */
+/* Both colours of one direction through a uniform per-direction base pointer
+ * and a 32-bit byte offset per lane (2 N pair bytes < 2^32, checked at
+ * create), so loads and stores use the SGPR-base + VGPR-offset address form
+ * instead of 64-bit VGPR addresses. */
 template <class S>
-__device__ __forceinline__ float ldq(const S* base, uint32_t node, float shift) {
-  return Store<S>::ld((const S*)((const char*)base + (size_t)(node * (uint32_t)sizeof(S))), 0, shift);
+__device__ __forceinline__ const S* dbase(const S* p, int l, size_t N) {
+  return p + 2 * (size_t)l * N;
 }
 template <class S>
-__device__ __forceinline__ void stq(S* base, uint32_t node, float v, float shift) {
-  Store<S>::st((S*)((char*)base + (size_t)(node * (uint32_t)sizeof(S))), 0, v, shift);
+__device__ __forceinline__ S* dbase(S* p, int l, size_t N) {
+  return p + 2 * (size_t)l * N;
 }
 template <class S>
-__device__ __forceinline__ void cpq(S* dbase, uint32_t dnode, const S* sbase, uint32_t snode) {
-  *(S*)((char*)dbase + (size_t)(dnode * (uint32_t)sizeof(S))) =
-      *(const S*)((const char*)sbase + (size_t)(snode * (uint32_t)sizeof(S)));
+__device__ __forceinline__ void ldq(const S* base, uint32_t node, float shift, float& r, float& b) {
+  Pair<S>::ld((const S*)((const char*)base + (size_t)(node * (uint32_t)(2 * sizeof(S)))), 0, shift, r, b);
+}
+template <class S>
+__device__ __forceinline__ void stq(S* base, uint32_t node, float r, float b, float shift) {
+  Pair<S>::st((S*)((char*)base + (size_t)(node * (uint32_t)(2 * sizeof(S)))), 0, r, b, shift);
 }
 
 /* rho = f_0 + f_1 + ... + f_18 in this order (init_rhoR_CG, :427-436) */
@@ -68,10 +72,7 @@ template <class S>
 __device__ __forceinline__ void load_sum(const S* src, uint32_t n, size_t N, float rho0, float (&fr)[19],
                                          float (&fb)[19], float& rr, float& rb) {
 #pragma unroll
-  for (int l = 0; l < 19; ++l) {
-    fr[l] = ldq(src + (size_t)l * N, n, W(l) * rho0);
-    fb[l] = ldq(src + (size_t)(19 + l) * N, n, W(l) * rho0);
-  }
+  for (int l = 0; l < 19; ++l) ldq(dbase(src, l, N), n, W(l) * rho0, fr[l], fb[l]);
   rr = fr[0];
   rb = fb[0];
 #pragma unroll
@@ -269,15 +270,10 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
          the float round trip (h + s) - s rounds back to h whenever
          |h + s| < 0.5, which the shift s = w_l rho0 with rho0 <= 1.2
          (checked at create) and stored deviations |h| < 0.1 satisfy. */
-      stq(dst, n, fr[0], W(0) * rho0);
-      stq(dst + (size_t)19 * N, n, fb[0], W(0) * rho0);
+      stq(dbase(dst, 0, N), n, fr[0], fb[0], W(0) * rho0);
       if (!PART || f != FL_DEAD) {
 #pragma unroll
-        for (int l = 1; l < 19; ++l) {
-          const uint32_t tn = nb3(EX(l), EY(l), EZ(l));
-          stq(dst + (size_t)l * N, tn, fr[l], W(l) * rho0);
-          stq(dst + (size_t)(19 + l) * N, tn, fb[l], W(l) * rho0);
-        }
+        for (int l = 1; l < 19; ++l) stq(dbase(dst, l, N), nb3(EX(l), EY(l), EZ(l)), fr[l], fb[l], W(l) * rho0);
       }
     }
     __syncthreads();

@@ -184,24 +184,18 @@ __device__ void omega_lab(const double* pos, int p, double o[3]) {
   o[2] = r[3];
 }
 
-template <class S>
-__device__ __forceinline__ float ldp(const void* p, size_t i, int l, float rho0) {
-  return Store<S>::ld((const S*)p, i, W(l) * rho0);
-}
-template <class S>
-__device__ __forceinline__ void stp(void* p, size_t i, int l, float v, float rho0) {
-  Store<S>::st((S*)p, i, v, W(l) * rho0);
-}
-
 /* rho_k = f_0 + ... + f_18 (init_rhoR_CG order, :427-436), clamped at
  * MINDENS = 0 (:443-446) */
 template <class S>
 __device__ void rho_sum(const void* pops, size_t N, size_t n, float rho0, float* rr, float* rb) {
-  float r = ldp<S>(pops, n, 0, rho0), b = ldp<S>(pops, (size_t)19 * N + n, 0, rho0);
+  float r, b;
+  ld_pair<S>(pops, N, 0, n, rho0, r, b);
 #pragma unroll
   for (int l = 1; l < 19; ++l) {
-    r = r + ldp<S>(pops, (size_t)l * N + n, l, rho0);
-    b = b + ldp<S>(pops, (size_t)(19 + l) * N + n, l, rho0);
+    float pr, pb;
+    ld_pair<S>(pops, N, l, n, rho0, pr, pb);
+    r = r + pr;
+    b = b + pb;
   }
   *rr = r < 0.0f ? 0.0f : r;
   *rb = b < 0.0f ? 0.0f : b;
@@ -281,10 +275,7 @@ __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
     const float inv = 1.0f / (rR + rB);
     float fr[19], fb[19];
 #pragma unroll
-    for (int l = 0; l < 19; ++l) {
-      fr[l] = ldp<S>(a.pop_src, (size_t)l * a.N + n, l, a.rho0);
-      fb[l] = ldp<S>(a.pop_src, (size_t)(19 + l) * a.N + n, l, a.rho0);
-    }
+    for (int l = 0; l < 19; ++l) ld_pair<S>(a.pop_src, a.N, l, n, a.rho0, fr[l], fb[l]);
     const float u = inv * (fr[1] - fr[2] + fr[7] - fr[8] - fr[9] + fr[10] + fr[11] - fr[12] - fr[13] + fr[14]) +
                     inv * (fb[1] - fb[2] + fb[7] - fb[8] - fb[9] + fb[10] + fb[11] - fb[12] - fb[13] + fb[14]);
     const float v = inv * (fr[3] - fr[4] + fr[7] - fr[8] + fr[9] - fr[10] + fr[15] - fr[16] - fr[17] + fr[18]) +
@@ -373,8 +364,7 @@ __device__ __forceinline__ void k_fill_apply_body(const PartArgs& a, int p) {
       for (int l = 0; l < 19; ++l) {
         const float uv = (1.0f / CSSQ) * (vx * (float)EX(l) + vy * (float)EY(l) + vz * (float)EZ(l));
         const float brk = 1.0f + uv + 0.5f * (uv * uv) - usq;
-        stp<S>(a.pop_src, (size_t)l * a.N + n, l, Rs * W(l) * brk, a.rho0);
-        stp<S>(a.pop_src, (size_t)(19 + l) * a.N + n, l, Bs * W(l) * brk, a.rho0);
+        st_pair<S>(a.pop_src, a.N, l, n, Rs * W(l) * brk, Bs * W(l) * brk, a.rho0);
       }
     }
     double ft[3] = {(double)(-(Rs + Bs) * vx), (double)(-(Rs + Bs) * vy), (double)(-(Rs + Bs) * vz)};
@@ -557,8 +547,7 @@ struct SurfaceNode {
       f[l] = a.fl_cur[nb[l]];
       rr[l] = a.rho_r[nb[l]];
       rb[l] = a.rho_b[nb[l]];
-      pr[l] = ldp<S>(a.pop_dst, (size_t)l * a.N + s, l, a.rho0);
-      pb[l] = ldp<S>(a.pop_dst, (size_t)(19 + l) * a.N + s, l, a.rho0);
+      ld_pair<S>(a.pop_dst, a.N, l, s, a.rho0, pr[l], pb[l]);
     }
   }
 };
@@ -670,8 +659,7 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
         fR = fR - W(lout) * BUZ * sn.rr[lin] * ev;
         fB = fB - W(lout) * BUZ * sn.rb[lin] * ev;
       }
-      stp<S>(a.pop_dst, (size_t)lin * a.N + sn.nb[lin], lin, fR, a.rho0);
-      stp<S>(a.pop_dst, (size_t)(19 + lin) * a.N + sn.nb[lin], lin, fB, a.rho0);
+      st_pair<S>(a.pop_dst, a.N, lin, sn.nb[lin], fR, fB, a.rho0);
     }
     link_force<S>(a, nc, kp, sn, gx, gy, gz, own);
   }
