@@ -432,9 +432,8 @@ int lbm_step(lbm_ctx* c, int32_t nsteps) {
     const bool fused = c->variant == LBM_KERNEL_FUSED;
     if (!fused) rc |= lbm_launch_density(c);
     if (parts) {
-      if (fused) rc |= lbm_particles_shell(c);
       rc |= lbm_particles_fill(c);
-      rc |= lbm_particles_wetting(c);
+      rc |= fused ? lbm_particles_shell_wetting(c) : lbm_particles_wetting(c);
     }
     rc |= fused ? lbm_launch_fused(c) : lbm_launch_collide(c);
     if (parts) {

@@ -98,6 +98,12 @@ template <> struct Store<__hip_bfloat16> {
  * direction, element by element exactly as Store<S> does. */
 template <class S> struct Pair;
 template <> struct Pair<float> {
+  using Raw = float2;   /* a pair as loaded, before conversion */
+  __device__ static Raw raw(const float* p, size_t i) { return reinterpret_cast<const float2*>(p)[i]; }
+  __device__ static void cvt(Raw v, float, float& r, float& b) {
+    r = v.x;
+    b = v.y;
+  }
   __device__ static void ld(const float* p, size_t i, float, float& r, float& b) {
     const float2 v = reinterpret_cast<const float2*>(p)[i];
     r = v.x;
@@ -108,6 +114,12 @@ template <> struct Pair<float> {
   }
 };
 template <> struct Pair<__half> {
+  using Raw = __half2;
+  __device__ static Raw raw(const __half* p, size_t i) { return reinterpret_cast<const __half2*>(p)[i]; }
+  __device__ static void cvt(Raw v, float s, float& r, float& b) {
+    r = __low2float(v) + s;
+    b = __high2float(v) + s;
+  }
   __device__ static void ld(const __half* p, size_t i, float s, float& r, float& b) {
     const __half2 v = reinterpret_cast<const __half2*>(p)[i];
     r = __low2float(v) + s;
@@ -118,6 +130,12 @@ template <> struct Pair<__half> {
   }
 };
 template <> struct Pair<__hip_bfloat16> {
+  using Raw = uint32_t;
+  __device__ static Raw raw(const __hip_bfloat16* p, size_t i) { return reinterpret_cast<const uint32_t*>(p)[i]; }
+  __device__ static void cvt(Raw u, float s, float& r, float& b) {
+    r = __uint_as_float(u << 16) + s;
+    b = __uint_as_float(u & 0xffff0000u) + s;
+  }
   __device__ static void ld(const __hip_bfloat16* p, size_t i, float s, float& r, float& b) {
     const uint32_t u = reinterpret_cast<const uint32_t*>(p)[i];
     r = __uint_as_float(u << 16) + s;
@@ -316,7 +334,7 @@ int lbm_particles_flags(lbm_ctx* c, int level);
 int lbm_particles_erase(lbm_ctx* c);
 int lbm_particles_fill(lbm_ctx* c);
 int lbm_particles_wetting(lbm_ctx* c);
-int lbm_particles_shell(lbm_ctx* c);
+int lbm_particles_shell_wetting(lbm_ctx* c);
 int lbm_particles_max_neighbours();
 int lbm_particles_bounce(lbm_ctx* c);
 int lbm_particles_integrate(lbm_ctx* c);

@@ -67,6 +67,13 @@ __device__ __forceinline__ void stq(S* base, uint32_t node, float r, float b, fl
   Pair<S>::st((S*)((char*)base + (size_t)(node * (uint32_t)(2 * sizeof(S)))), 0, r, b, shift);
 }
 
+/* the pair of direction l at node n as loaded (16-bit pairs stay packed in
+ * one register until they are converted) */
+template <class S>
+__device__ __forceinline__ typename Pair<S>::Raw ldraw(const S* src, int l, size_t N, uint32_t n) {
+  return Pair<S>::raw((const S*)((const char*)dbase(src, l, N) + (size_t)(n * (uint32_t)(2 * sizeof(S)))), 0);
+}
+
 /* rho = f_0 + f_1 + ... + f_18 in this order (init_rhoR_CG, :427-436) */
 template <class S>
 __device__ __forceinline__ void load_sum(const S* src, uint32_t n, size_t N, float rho0, float (&fr)[19],
@@ -87,6 +94,49 @@ __device__ __forceinline__ void sum_only(const S* src, uint32_t n, size_t N, flo
   float fr[19], fb[19];
   load_sum(src, n, N, rho0, fr, fb, rr, rb);
 }
+
+/* The populations of the next plane, kept in registers across the barrier:
+ * converted floats for float storage; for 16-bit storage the packed pairs as
+ * loaded (19 registers instead of 38), converted again when used. */
+template <class S, bool PACKED = (sizeof(S) == 2)>
+struct Ahead;
+template <class S>
+struct Ahead<S, false> {
+  float r[19], b[19];
+  __device__ __forceinline__ void load(const S* src, uint32_t n, size_t N, float rho0, float& rr, float& rb) {
+    load_sum(src, n, N, rho0, r, b, rr, rb);
+  }
+  __device__ __forceinline__ void get(float, float (&fr)[19], float (&fb)[19]) const {
+#pragma unroll
+    for (int l = 0; l < 19; ++l) {
+      fr[l] = r[l];
+      fb[l] = b[l];
+    }
+  }
+};
+template <class S>
+struct Ahead<S, true> {
+  typename Pair<S>::Raw q[19];
+  __device__ __forceinline__ void load(const S* src, uint32_t n, size_t N, float rho0, float& rr, float& rb) {
+    float fr[19], fb[19];
+#pragma unroll
+    for (int l = 0; l < 19; ++l) {
+      q[l] = ldraw(src, l, N, n);
+      Pair<S>::cvt(q[l], W(l) * rho0, fr[l], fb[l]);
+    }
+    rr = fr[0];
+    rb = fb[0];
+#pragma unroll
+    for (int l = 1; l < 19; ++l) {
+      rr = rr + fr[l];
+      rb = rb + fb[l];
+    }
+  }
+  __device__ __forceinline__ void get(float rho0, float (&fr)[19], float (&fb)[19]) const {
+#pragma unroll
+    for (int l = 0; l < 19; ++l) Pair<S>::cvt(q[l], W(l) * rho0, fr[l], fb[l]);
+  }
+};
 
 template <int TX, int TY, bool WX>
 struct TileGeom {
@@ -145,12 +195,12 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
                           (uint32_t)(y == ny - 1 ? 0 : y + 1) * (uint32_t)nx};
 
   /* state of the interior node on the next plane (kept in registers) */
-  float qr[19], qb[19];
+  Ahead<S> ahead;
   float qR = 0.0f, qB = 0.0f;
   uint8_t qF = FL_FLUID;
 
   /* phi of plane zl (wrapped) into ring slot s; the halo node first, then the
-   * interior node, whose populations are kept in qr/qb when keep is set.
+   * interior node, whose populations are kept in ahead when keep is set.
    * With particles the node type and the stored phase field (zero densities
    * inside particles, wetting values on their surface) are loaded beside the
    * populations, not before them, so no load waits on another. */
@@ -172,7 +222,7 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
       qF = PART ? a.flags[n] : FL_FLUID;
       const float qp = PART ? a.phi[n] : 0.0f;
       float rr, rb;
-      if (keep) load_sum(src, n, N, rho0, qr, qb, rr, rb);
+      if (keep) ahead.load(src, n, N, rho0, rr, rb);
       else sum_only(src, n, N, rho0, rr, rb);
       float ph = qp;
       if (qF == FL_FLUID) {
@@ -193,8 +243,7 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
   int sm = 0, s0 = 1, sp = 2;
   for (int z = z0; z < z1; ++z) {
     float fr[19], fb[19];
-#pragma unroll
-    for (int l = 0; l < 19; ++l) { fr[l] = qr[l]; fb[l] = qb[l]; }
+    ahead.get(rho0, fr, fb);
     const float rR = qR, rB = qB;
     const uint8_t f = qF;
     plane(z + 1, sp, z + 1 < z1);

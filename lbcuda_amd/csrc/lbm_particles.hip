@@ -27,12 +27,6 @@ namespace {
 constexpr int TPB = 256;
 constexpr int MAXNB = 32;   /* neighbour-list capacity per particle */
 
-/* D3Q27 links in the reference order (dimensions_m.CUF:275-282) */
-__constant__ int8_t C_E27[27][3] = {
-    {0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},  {0, 0, 1},  {0, 0, -1},
-    {1, 1, 0},   {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}, {1, 0, 1},   {-1, 0, -1}, {-1, 0, 1},
-    {1, 0, -1},  {0, 1, 1},   {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {1, 1, 1},  {-1, -1, -1},
-    {1, 1, -1},  {-1, -1, 1}, {-1, 1, 1}, {1, -1, -1}, {1, -1, 1}, {-1, 1, -1}};
 /* D3Q27 weights pd3q27 */
 __device__ __forceinline__ float PD27(int l) { return l == 0 ? P0D : (l < 7 ? P1D : (l < 19 ? P2D : P3D)); }
 
@@ -327,17 +321,41 @@ __device__ __forceinline__ void k_fill_apply_body(const PartArgs& a, int p) {
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
     if (!fill_candidate(a, n)) continue;
     const uint8_t fo = a.fl_old[n];
-    /* compute_onebelt_density_twofluids, PART:783-813 */
+    /* compute_onebelt_density_twofluids, PART:783-813: the neighbours' types
+       are loaded in one batch, then the densities of the neighbours fluid on
+       both levels, two at a time, summed in link order */
     float Rs = 0.0f, Bs = 0.0f;
     int D = 0;
-    for (int l = 1; l < 27; ++l) {
-      size_t m = wrap_node(a, gx + C_E27[l][0], gy + C_E27[l][1], gz + C_E27[l][2]);
-      if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
-        float mr, mb;
-        belt_rho<S>(a, m, &mr, &mb);
-        Rs = Rs + mr;
-        Bs = Bs + mb;
+    {
+      uint32_t mn[26];
+      uint8_t fc[26], fp[26];
+#pragma unroll
+      for (int l = 1; l < 27; ++l) mn[l - 1] = (uint32_t)wrap_node(a, gx + EX27(l), gy + EY27(l), gz + EZ27(l));
+#pragma unroll
+      for (int l = 0; l < 26; ++l) {
+        fc[l] = a.fl_cur[mn[l]];
+        fp[l] = a.fl_old[mn[l]];
+      }
+      uint32_t mask = 0;
+#pragma unroll
+      for (int l = 0; l < 26; ++l)
+        if (fc[l] == FL_FLUID && fp[l] == FL_FLUID) mask |= 1u << l;
+      while (mask) {
+        const int l1 = __ffs(mask) - 1;
+        mask &= mask - 1;
+        const int l2 = mask ? __ffs(mask) - 1 : -1;
+        if (l2 >= 0) mask &= mask - 1;
+        float r1, b1, r2 = 0.0f, b2 = 0.0f;
+        belt_rho<S>(a, mn[l1], &r1, &b1);
+        if (l2 >= 0) belt_rho<S>(a, mn[l2], &r2, &b2);
+        Rs = Rs + r1;
+        Bs = Bs + b1;
         D = D + 1;
+        if (l2 >= 0) {
+          Rs = Rs + r2;
+          Bs = Bs + b2;
+          D = D + 1;
+        }
       }
     }
     if (D == 0) {
@@ -395,14 +413,14 @@ __device__ __forceinline__ void k_fill_release_body(const PartArgs& a, int p) {
 }
 
 /* ---------------- wetting of surface nodes -------------------------------- */
+/* densities of surface node t of the list of particle p (centre c), if p owns
+ * it: Janus or neutral wetting */
 template <class S>
-__device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
-  int c[3];
-  centre(a.pos_cur, p, c);
-  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntlist; t += TPB * gridDim.y) {
+__device__ __forceinline__ void wet_node(const PartArgs& a, int p, const int c[3], int t) {
+  {
     int gx, gy, gz;
     size_t n = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
-    if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) continue;
+    if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) return;
     float rr, rb;
     if (a.wetting == LBM_WET_JANUS) {
       /* compute_virtual_rho, PART:299-352: sign of r.(q x q*) */
@@ -456,7 +474,7 @@ __device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
             D = D + 1;
           }
         }
-        if (D == 0) continue;   /* fix_onebelt_density_twofluids is empty */
+        if (D == 0) return;   /* fix_onebelt_density_twofluids is empty */
         rr = R2 / (float)D;
         rb = B2 / (float)D;
       }
@@ -465,6 +483,13 @@ __device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
     a.rho_b[n] = rb;
     a.phi[n] = (rr - rb) / (rr + rb);
   }
+}
+
+template <class S>
+__device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
+  int c[3];
+  centre(a.pos_cur, p, c);
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntlist; t += TPB * gridDim.y) wet_node<S>(a, p, c, t);
 }
 
 /* ---------------- densities around each particle (single-pass step) -----
@@ -476,29 +501,44 @@ __device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
  * particle's own non-fluid nodes MINDENS = 0 (init_rhoR_CG / init_rhoB_CG,
  * kernels_fluid_CG.CUF:398-509).  Offsets: the template and the D3Q27 ring
  * of its surface nodes. */
+/* the particle's own template node t: MINDENS (every template node is
+ * non-fluid) */
+__device__ __forceinline__ void shell_zero(const PartArgs& a, int p, const int c[3], int t) {
+  const size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
+  if ((a.key_cur[n] >> 3) != (uint32_t)p) return;
+  a.rho_r[n] = 0.0f;
+  a.rho_b[n] = 0.0f;
+  a.phi[n] = (0.0f - 0.0f) / (0.0f + 0.0f);
+}
+
+/* ring node t around the particle's surface: densities of a fluid node from
+ * this step's populations.  A node filled this step (fluid now, not on the
+ * previous level) keeps the belt densities make_fluid gave it. */
 template <class S>
-__device__ __forceinline__ void k_shell_body(const PartArgs& a, int p) {
+__device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], int t) {
+  const size_t n = node_at(a, c[0], c[1], c[2], a.ring[4 * t], a.ring[4 * t + 1], a.ring[4 * t + 2]);
+  const uint8_t f = a.fl_cur[n], fo = a.fl_old[n];
+  float rr, rb;
+  rho_sum<S>(a.pop_src, a.N, n, a.rho0, &rr, &rb);   /* issued with the type loads */
+  if (f != FL_FLUID || fo != FL_FLUID) return;
+  a.rho_r[n] = rr;
+  a.rho_b[n] = rb;
+  a.phi[n] = (rr - rb) / (rr + rb);
+}
+
+/* shell densities and wetting in one pass, one workgroup per particle (runs
+ * after make_fluid).  Every fluid D3Q27 neighbour of a surface node p owns is
+ * in p's ring, so the wetting after the barrier reads only densities this
+ * workgroup wrote (or make_fluid wrote before the launch). */
+template <class S>
+__device__ __forceinline__ void k_shellwet_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
-  /* the particle's own nodes: MINDENS (every template node is non-fluid) */
-  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
-    const size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
-    if ((a.key_cur[n] >> 3) != (uint32_t)p) continue;
-    a.rho_r[n] = 0.0f;
-    a.rho_b[n] = 0.0f;
-    a.phi[n] = (0.0f - 0.0f) / (0.0f + 0.0f);
-  }
-  /* the ring around its surface: densities of the fluid nodes */
-  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.nring; t += TPB * gridDim.y) {
-    const size_t n = node_at(a, c[0], c[1], c[2], a.ring[4 * t], a.ring[4 * t + 1], a.ring[4 * t + 2]);
-    const uint8_t f = a.fl_cur[n];
-    float rr, rb;
-    rho_sum<S>(a.pop_src, a.N, n, a.rho0, &rr, &rb);   /* issued with the type load */
-    if (f != FL_FLUID) continue;
-    a.rho_r[n] = rr;
-    a.rho_b[n] = rb;
-    a.phi[n] = (rr - rb) / (rr + rb);
-  }
+  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) shell_zero(a, p, c, t);
+  for (int t = threadIdx.x; t < a.nring; t += TPB) shell_ring<S>(a, c, t);
+  if (a.wetting == LBM_WET_NONE) return;
+  __syncthreads();
+  for (int t = threadIdx.x; t < a.ntlist; t += TPB) wet_node<S>(a, p, c, t);
 }
 
 /* ---------------- bounce-back on surface links ---------------------------
@@ -532,9 +572,11 @@ __device__ __forceinline__ int ball_r2(const PartArgs& a, const int c[3], int gx
   return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
 }
 
-/* what the link pass reads around one surface node s after the collide:
- * the type and densities of its 18 D3Q19 neighbours and the 18 values the
- * neighbours pushed into s (dst(s, l) comes from s - e_l), in one batch */
+/* what the link pass reads around one surface node s after the collide: the
+ * types of its 18 D3Q19 neighbours (one batch), then for each link, i.e.
+ * each opposite pair (lo, hi) with exactly one fluid neighbour ii = s +
+ * e_lin, the densities of ii and the value ii pushed into s, dst(s, lout)
+ * (second batch; directions without a link are not read) */
 template <class S>
 struct SurfaceNode {
   size_t nb[19];
@@ -545,9 +587,17 @@ struct SurfaceNode {
     for (int l = 1; l < 19; ++l) {
       nb[l] = wrap_node(a, gx + EX(l), gy + EY(l), gz + EZ(l));
       f[l] = a.fl_cur[nb[l]];
-      rr[l] = a.rho_r[nb[l]];
-      rb[l] = a.rho_b[nb[l]];
-      ld_pair<S>(a.pop_dst, a.N, l, s, a.rho0, pr[l], pb[l]);
+    }
+#pragma unroll
+    for (int il = 1; il <= 9; ++il) {
+      const int lo = 2 * il - 1, hi = 2 * il;
+      const bool fi = f[lo] == FL_FLUID, fo = f[hi] == FL_FLUID;
+      if (fi == fo) continue;
+      const int lin = fi ? lo : hi;
+      const int lout = fi ? hi : lo;
+      rr[lin] = a.rho_r[nb[lin]];
+      rb[lin] = a.rho_b[nb[lin]];
+      ld_pair<S>(a.pop_dst, a.N, lout, s, a.rho0, pr[lout], pb[lout]);
     }
   }
 };
@@ -994,7 +1044,7 @@ LBM_PART_KERNEL(k_neigh)
 LBM_PART_KERNEL_S(k_erase)
 LBM_PART_KERNEL_S(k_fill_apply)
 LBM_PART_KERNEL_S(k_wetting)
-LBM_PART_KERNEL_S(k_shell)
+LBM_PART_KERNEL_S(k_shellwet)
 LBM_PART_KERNEL_S(k_links)
 
 __global__ __launch_bounds__(TPB) void k_pairs(PartArgs a, PairParams q) { k_pairs_body(a, blockIdx.x, q); }
@@ -1123,10 +1173,9 @@ int lbm_particles_wetting(lbm_ctx* c) {
   return last_err();
 }
 
-int lbm_particles_shell(lbm_ctx* c) {
+int lbm_particles_shell_wetting(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
-  const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
-  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shell<S>, g, dim3(TPB), 0, c->stream, a))
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shellwet<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
   return last_err();
 }
 
@@ -1247,7 +1296,7 @@ void lbm_part_batch_destroy(lbm_part_batch* b) {
 
 int lbm_part_batch_count(const lbm_part_batch* b) { return b->nitems; }
 
-/* flags, erase, shell densities, fill, wetting (the particle work before the
+/* flags, erase, fill, shell densities + wetting (the particle work before the
  * collide-stream kernel, in lbm_step's order) */
 int lbm_part_batch_pre(lbm_part_batch* b, int parity, long step, hipStream_t st) {
   if (b->nitems == 0) return 0;
@@ -1261,11 +1310,10 @@ int lbm_part_batch_pre(lbm_part_batch* b, int parity, long step, hipStream_t st)
   hipLaunchKernelGGL(k_resolve_batch, gc, t, 0, st, A, b->d_items, bs);
   LBM_STORAGE_SWITCH(b->storage,
     hipLaunchKernelGGL(k_erase_batch<S>, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_shell_batch<S>, gc, t, 0, st, A, b->d_items, bs);
     hipLaunchKernelGGL(k_fill_claim_batch, gc, t, 0, st, A, b->d_items, bs);
     hipLaunchKernelGGL(k_fill_apply_batch<S>, g, t, 0, st, A, b->d_items, bs);
     hipLaunchKernelGGL(k_fill_release_batch, gc, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_wetting_batch<S>, gc, t, 0, st, A, b->d_items, bs))
+    hipLaunchKernelGGL(k_shellwet_batch<S>, g, t, 0, st, A, b->d_items, bs))
   return last_err();
 }
 
