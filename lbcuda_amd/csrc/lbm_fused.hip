@@ -40,46 +40,105 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 using namespace lbm;
 
 namespace {
 
-/* Both colours of one direction through a uniform per-direction base pointer
- * and a 32-bit byte offset per lane (2 N pair bytes < 2^32, checked at
- * create), so loads and stores use the SGPR-base + VGPR-offset address form
- * instead of 64-bit VGPR addresses. */
+/* Population access.  A time level is addressed through five buffer
+ * resources, one per group of four direction planes (base = level + 4 g *
+ * plane bytes), a uniform offset (l mod 4) * plane bytes and a 32-bit byte
+ * offset per lane, so a load or store needs no vector address arithmetic.
+ * Offsets within a group stay below 4 GiB because four planes do (checked at
+ * create).  A pair is 8 bytes (float) or 4 bytes (16-bit, kept packed in one
+ * register until it is converted). */
+using Rsrc = __amdgpu_buffer_rsrc_t;
+using V2U = __attribute__((__vector_size__(2 * sizeof(unsigned int)))) unsigned int;
+
 template <class S>
-__device__ __forceinline__ const S* dbase(const S* p, int l, size_t N) {
-  return p + 2 * (size_t)l * N;
-}
+struct LevelIO {
+  static constexpr uint32_t PS = 2 * sizeof(S);   /* bytes of one pair */
+  Rsrc r[5];
+  uint32_t pb;                                    /* bytes of one direction plane */
+  __device__ __forceinline__ LevelIO(const void* level, uint32_t plane_bytes) : pb(plane_bytes) {
+    char* base = const_cast<char*>(static_cast<const char*>(level));
+#pragma unroll
+    for (int g = 0; g < 5; ++g)
+      r[g] = __builtin_amdgcn_make_buffer_rsrc(base + (size_t)(4 * g) * plane_bytes, (short)0, -1, 0x00020000);
+  }
+  __device__ __forceinline__ typename Pair<S>::Raw ld(int l, uint32_t node) const {
+    const uint32_t so = (uint32_t)(l & 3) * pb;
+    if constexpr (sizeof(S) == 4)
+      return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r[l >> 2], node * PS, so, 0));
+    else
+      return __builtin_bit_cast(typename Pair<S>::Raw,
+                                __builtin_amdgcn_raw_buffer_load_b32(r[l >> 2], node * PS, so, 0));
+  }
+  /* store a pair given as floats (already shifted for 16-bit storage); the
+   * 16-bit conversions round to nearest even, two values per instruction */
+  __device__ __forceinline__ void st(int l, uint32_t node, float vr, float vb) const {
+    const uint32_t so = (uint32_t)(l & 3) * pb;
+    if constexpr (sizeof(S) == 4) {
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2U, make_float2(vr, vb)), r[l >> 2], node * PS, so, 0);
+    } else if constexpr (std::is_same<S, __half>::value) {
+      typedef _Float16 H2 __attribute__((ext_vector_type(2)));
+      const H2 h = {(_Float16)vr, (_Float16)vb};
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, h), r[l >> 2], node * PS, so, 0);
+    } else {
+      typedef __bf16 B2 __attribute__((ext_vector_type(2)));
+      const B2 h = {(__bf16)vr, (__bf16)vb};
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, h), r[l >> 2], node * PS, so, 0);
+    }
+  }
+};
+
+/* convert a loaded pair to float (no shift) */
 template <class S>
-__device__ __forceinline__ S* dbase(S* p, int l, size_t N) {
-  return p + 2 * (size_t)l * N;
-}
-template <class S>
-__device__ __forceinline__ void ldq(const S* base, uint32_t node, float shift, float& r, float& b) {
-  Pair<S>::ld((const S*)((const char*)base + (size_t)(node * (uint32_t)(2 * sizeof(S)))), 0, shift, r, b);
-}
-template <class S>
-__device__ __forceinline__ void stq(S* base, uint32_t node, float r, float b, float shift) {
-  Pair<S>::st((S*)((char*)base + (size_t)(node * (uint32_t)(2 * sizeof(S)))), 0, r, b, shift);
+__device__ __forceinline__ void cvt_raw(typename Pair<S>::Raw q, float& r, float& b) {
+  if constexpr (sizeof(S) == 4) {
+    r = q.x;
+    b = q.y;
+  } else if constexpr (std::is_same<S, __half>::value) {
+    r = __low2float(q);
+    b = __high2float(q);
+  } else {
+    r = __uint_as_float(q << 16);
+    b = __uint_as_float(q & 0xffff0000u);
+  }
 }
 
-/* the pair of direction l at node n as loaded (16-bit pairs stay packed in
- * one register until they are converted) */
-template <class S>
-__device__ __forceinline__ typename Pair<S>::Raw ldraw(const S* src, int l, size_t N, uint32_t n) {
-  return Pair<S>::raw((const S*)((const char*)dbase(src, l, N) + (size_t)(n * (uint32_t)(2 * sizeof(S)))), 0);
+/* reciprocal and reciprocal square root to 1 ulp (v_rcp_f32 / v_rsq_f32)
+ * instead of the correctly rounded multi-instruction sequences */
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+/* e_l . (x, y, z) with no multiplications by zero components (0 * x is not
+ * folded away in IEEE arithmetic) */
+__device__ __forceinline__ float edot(int l, float x, float y, float z) {
+  float r = 0.0f;
+  bool any = false;
+  if (EX(l)) { r = EX(l) > 0 ? x : -x; any = true; }
+  if (EY(l)) { r = any ? (EY(l) > 0 ? r + y : r - y) : (EY(l) > 0 ? y : -y); any = true; }
+  if (EZ(l)) { r = any ? (EZ(l) > 0 ? r + z : r - z) : (EZ(l) > 0 ? z : -z); }
+  return r;
 }
 
-/* rho = f_0 + f_1 + ... + f_18 in this order (init_rhoR_CG, :427-436) */
+/* 16-bit storage holds h = f - w_l rho0 (the shift makes the stored
+ * deviations small); the fused kernel works on h directly and folds the
+ * shift into its per-node constants, so no per-value add or subtract is spent
+ * on it.  Float storage holds f (no shift). */
 template <class S>
-__device__ __forceinline__ void load_sum(const S* src, uint32_t n, size_t N, float rho0, float (&fr)[19],
+constexpr bool kShifted = sizeof(S) == 2;
+
+/* rho = f_0 + f_1 + ... + f_18 in this order (init_rhoR_CG, :427-436); for
+ * shifted storage the sum of the h, plus sum_l w_l rho0 = rho0 */
+template <class S>
+__device__ __forceinline__ void load_sum(const LevelIO<S>& src, uint32_t n, float rho0, float (&fr)[19],
                                          float (&fb)[19], float& rr, float& rb) {
 #pragma unroll
-  for (int l = 0; l < 19; ++l) ldq(dbase(src, l, N), n, W(l) * rho0, fr[l], fb[l]);
+  for (int l = 0; l < 19; ++l) cvt_raw<S>(src.ld(l, n), fr[l], fb[l]);
   rr = fr[0];
   rb = fb[0];
 #pragma unroll
@@ -87,26 +146,30 @@ __device__ __forceinline__ void load_sum(const S* src, uint32_t n, size_t N, flo
     rr = rr + fr[l];
     rb = rb + fb[l];
   }
+  if (kShifted<S>) {
+    rr = rr + rho0;
+    rb = rb + rho0;
+  }
 }
 
 template <class S>
-__device__ __forceinline__ void sum_only(const S* src, uint32_t n, size_t N, float rho0, float& rr, float& rb) {
+__device__ __forceinline__ void sum_only(const LevelIO<S>& src, uint32_t n, float rho0, float& rr, float& rb) {
   float fr[19], fb[19];
-  load_sum(src, n, N, rho0, fr, fb, rr, rb);
+  load_sum<S>(src, n, rho0, fr, fb, rr, rb);
 }
 
 /* The populations of the next plane, kept in registers across the barrier:
- * converted floats for float storage; for 16-bit storage the packed pairs as
- * loaded (19 registers instead of 38), converted again when used. */
+ * floats for float storage; for 16-bit storage the packed pairs as loaded
+ * (19 registers instead of 38), converted to float h values when used. */
 template <class S, bool PACKED = (sizeof(S) == 2)>
 struct Ahead;
 template <class S>
 struct Ahead<S, false> {
   float r[19], b[19];
-  __device__ __forceinline__ void load(const S* src, uint32_t n, size_t N, float rho0, float& rr, float& rb) {
-    load_sum(src, n, N, rho0, r, b, rr, rb);
+  __device__ __forceinline__ void load(const LevelIO<S>& src, uint32_t n, float rho0, float& rr, float& rb) {
+    load_sum<S>(src, n, rho0, r, b, rr, rb);
   }
-  __device__ __forceinline__ void get(float, float (&fr)[19], float (&fb)[19]) const {
+  __device__ __forceinline__ void get(float (&fr)[19], float (&fb)[19]) const {
 #pragma unroll
     for (int l = 0; l < 19; ++l) {
       fr[l] = r[l];
@@ -117,12 +180,12 @@ struct Ahead<S, false> {
 template <class S>
 struct Ahead<S, true> {
   typename Pair<S>::Raw q[19];
-  __device__ __forceinline__ void load(const S* src, uint32_t n, size_t N, float rho0, float& rr, float& rb) {
+  __device__ __forceinline__ void load(const LevelIO<S>& src, uint32_t n, float rho0, float& rr, float& rb) {
     float fr[19], fb[19];
 #pragma unroll
     for (int l = 0; l < 19; ++l) {
-      q[l] = ldraw(src, l, N, n);
-      Pair<S>::cvt(q[l], W(l) * rho0, fr[l], fb[l]);
+      q[l] = src.ld(l, n);
+      cvt_raw<S>(q[l], fr[l], fb[l]);
     }
     rr = fr[0];
     rb = fb[0];
@@ -131,12 +194,97 @@ struct Ahead<S, true> {
       rr = rr + fr[l];
       rb = rb + fb[l];
     }
+    rr = rr + rho0;
+    rb = rb + rho0;
   }
-  __device__ __forceinline__ void get(float rho0, float (&fr)[19], float (&fb)[19]) const {
+  __device__ __forceinline__ void get(float (&fr)[19], float (&fb)[19]) const {
 #pragma unroll
-    for (int l = 0; l < 19; ++l) Pair<S>::cvt(q[l], W(l) * rho0, fr[l], fb[l]);
+    for (int l = 0; l < 19; ++l) cvt_raw<S>(q[l], fr[l], fb[l]);
   }
 };
+
+/* weight class of direction l: 0 rest, 1 axis, 2 diagonal */
+__device__ constexpr int WCLASS(int l) { return l == 0 ? 0 : (l < 7 ? 1 : 2); }
+__device__ constexpr float WK(int k) { return k == 0 ? W0 : (k == 1 ? W1 : W2); }
+__device__ constexpr float BK(int k) { return k == 0 ? B0 : (k == 1 ? B1 : B2); }
+
+/* Collision of one fluid node, time_step_CG (kernels_fluid_CG.CUF:689-846),
+ * in place on fr/fb: BGK towards f^eq(rho_k, u) per colour (:794-800), and
+ * where |grad phi|^2 > 1e-20 the Leclaire perturbation (:803-815) and the
+ * d'Ortona recolouring (:817-837).  The formulas are rearranged so that
+ * everything that depends only on the node is formed once and each
+ * direction costs about fifteen operations:
+ *   f^eq_l(rho_k) = rho_k w_l (c1 + uv (1 + uv / 2)), uv = 3 e_l.u, c1 = 1 - 3/2 |u|^2
+ *   perturbation  = (A w_l / |g|) (e_l.g)^2 - A |g| B_l, A = 9/4 omega sigma
+ *   recolouring   = fd rho_k / rho +- T_l (e_l.g), T_l = beta rho_R rho_B / rho w_l / (|e_l| |g|)
+ * (tt * feq of the reference, beta rho_R rho_B / rho^2 cos(phi_l) w_l rho, is
+ * the same product).  Shifted storage: fr/fb hold h on entry and the shifted
+ * outputs on exit; the shift enters the constants (BGK: -w_l rho0 omega;
+ * perturbation: +2 w_l rho0; recolouring: -w_l rho0).  The momentum is formed
+ * from the differences of opposite directions, in which the shift cancels.
+ * 1/rho, 1/nu, 1/tau and 1/|g| use the 1-ulp hardware reciprocals; the colour
+ * fractions rho_k / rho are correctly rounded quotients, since they carry each
+ * colour's mass through the recolouring. */
+template <bool SH>
+__device__ __forceinline__ void collide_node(float (&fr)[19], float (&fb)[19], float rR, float rB, float phf,
+                                             float px, float py, float pz, const FluidArgs& a) {
+  const float rs = rR + rB;
+  const float visc = frcp(0.5f * (1.0f + phf) * a.inv_visc_r + 0.5f * (1.0f - phf) * a.inv_visc_b);
+  const float omega = frcp(visc * 3.0f + 0.5f);
+  const float omm = 1.0f - omega;
+  const float inv = frcp(rs);
+  float d[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) d[i] = (fr[2 * i + 1] - fr[2 * i + 2]) + (fb[2 * i + 1] - fb[2 * i + 2]);
+  const float u = inv * (d[0] + d[3] - d[4] + d[5] - d[6]);
+  const float v = inv * (d[1] + d[3] + d[4] + d[7] - d[8]);
+  const float wv = inv * (d[2] + d[5] + d[6] + d[7] + d[8]);
+  const float u3 = 3.0f * u, v3 = 3.0f * v, w3 = 3.0f * wv;
+  const float c1 = 1.0f - (0.5f / CSSQ) * (u * u + v * v + wv * wv);
+  const float rRo = rR * omega, rBo = rB * omega;
+  const float swn = SH ? a.rho0 * omega : 0.0f;
+  /* BGK towards f^eq(rho_k, u) per colour (:794-800) */
+#pragma unroll
+  for (int l = 0; l < 19; ++l) {
+    const int k = WCLASS(l);
+    float brk = c1;
+    if (l > 0) {
+      const float uv = edot(l, u3, v3, w3);
+      brk = fmaf(uv, fmaf(0.5f, uv, 1.0f), c1);
+    }
+    const float bw = WK(k) * brk;
+    fr[l] = fmaf(fr[l], omm, SH ? fmaf(rRo, bw, -WK(k) * swn) : rRo * bw);
+    fb[l] = fmaf(fb[l], omm, SH ? fmaf(rBo, bw, -WK(k) * swn) : rBo * bw);
+  }
+  const float pn2 = px * px + py * py + pz * pz;
+  if (pn2 > 1.e-20f) {
+    /* perturbation (:803-815) and recolouring (:817-837) */
+    const float ipn = frsq(pn2);
+    const float pn = pn2 * ipn;
+    const float ac = 9.0f / 4.0f * omega * a.sigma;
+    const float acp = ac * ipn, acq = ac * pn;
+    const float tb = a.beta * rR * rB * inv * ipn;
+    const float rRi = rR / rs, rBi = rB / rs;
+    const float shn = SH ? a.rho0 : 0.0f;
+#pragma unroll
+    for (int l = 0; l < 19; ++l) {
+      const int k = WCLASS(l);
+      const float sk = WK(k) * shn;
+      const float q = SH ? fmaf(acq, BK(k), -2.0f * sk) : acq * BK(k);
+      if (l == 0) {
+        const float fd = (fr[0] + fb[0]) - q;
+        fr[0] = SH ? fmaf(fd, rRi, -sk) : fd * rRi;
+        fb[0] = SH ? fmaf(fd, rBi, -sk) : fd * rBi;
+      } else {
+        const float ed = edot(l, px, py, pz);
+        const float fd = (fr[l] + fb[l]) + fmaf(acp * WK(k), ed * ed, -q);
+        const float te = (tb * (k == 1 ? W1 : W2 * INV_SQRT2)) * ed;
+        fr[l] = fmaf(fd, rRi, SH ? te - sk : te);
+        fb[l] = fmaf(fd, rBi, SH ? -te - sk : -te);
+      }
+    }
+  }
+}
 
 template <int TX, int TY, bool WX>
 struct TileGeom {
@@ -153,11 +301,11 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
                                            float (&sphi)[3][TY + 2][TileGeom<TX, TY, WX>::LW]) {
   using G = TileGeom<TX, TY, WX>;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const size_t N = a.N;
+  const uint32_t pb = (uint32_t)(a.N * (2 * sizeof(S)));   /* bytes of one direction plane */
   const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
   const float rho0 = a.rho0;
-  const S* src = (const S*)a.src;
-  S* dst = (S*)a.dst;
+  const LevelIO<S> src(a.src, pb);
+  const LevelIO<S> dst(a.dst, pb);
   const int x0 = itx * TX, y0 = ity * TY, z0 = izc * zc;
   const int z1 = min(z0 + zc, nz);
   const int w = WX ? TX : min(TX, nx - x0);
@@ -200,11 +348,13 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
   uint8_t qF = FL_FLUID;
 
   /* phi of plane zl (wrapped) into ring slot s; the halo node first, then the
-   * interior node, whose populations are kept in ahead when keep is set.
-   * With particles the node type and the stored phase field (zero densities
-   * inside particles, wetting values on their surface) are loaded beside the
-   * populations, not before them, so no load waits on another. */
-  auto plane = [&](int zl, int s, bool keep) {
+   * interior node, whose populations are kept in ahead for the next
+   * iteration (after the last plane of the chunk they are not used).  With
+   * particles the node type and the stored phase field (zero densities inside
+   * particles, wetting values on their surface) are loaded beside the
+   * populations, and the phase field of a halo node is formed whatever its
+   * type and selected afterwards, so no population load waits on a type. */
+  auto plane = [&](int zl, int s) {
     const int zg = zl < 0 ? zl + nz : (zl >= nz ? zl - nz : zl);
     const uint32_t zoff = (uint32_t)zg * sxy;
     if (has_halo) {
@@ -212,25 +362,25 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
       const uint8_t hf = PART ? a.flags[n] : FL_FLUID;
       const float hp = PART ? a.phi[n] : 0.0f;
       float rr, rb;
-      sum_only(src, n, N, rho0, rr, rb);
+      sum_only<S>(src, n, rho0, rr, rb);
       if (rr < 0.0f) rr = 0.0f;
       if (rb < 0.0f) rb = 0.0f;
-      sphi[s][hrow][hcol] = hf == FL_FLUID ? (rr - rb) / (rr + rb) : hp;
+      const float ph = (rr - rb) * frcp(rr + rb);
+      sphi[s][hrow][hcol] = hf == FL_FLUID ? ph : hp;
     }
     if (inside) {
       const uint32_t n = nxy + zoff;
       qF = PART ? a.flags[n] : FL_FLUID;
       const float qp = PART ? a.phi[n] : 0.0f;
       float rr, rb;
-      if (keep) ahead.load(src, n, N, rho0, rr, rb);
-      else sum_only(src, n, N, rho0, rr, rb);
+      ahead.load(src, n, rho0, rr, rb);
       float ph = qp;
       if (qF == FL_FLUID) {
         if (rr < a.min_rho || rr > a.max_rho || rb < a.min_rho || rb > a.max_rho)
           raise_error(a.status, LBM_ERR_RHO_RANGE, a.step);
         if (rr < 0.0f) rr = 0.0f;
         if (rb < 0.0f) rb = 0.0f;
-        ph = (rr - rb) / (rr + rb);
+        ph = (rr - rb) * frcp(rr + rb);
       }
       qR = rr;
       qB = rb;
@@ -238,15 +388,15 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
     }
   };
 
-  plane(z0 - 1, 0, false);
-  plane(z0, 1, true);
+  plane(z0 - 1, 0);
+  plane(z0, 1);
   int sm = 0, s0 = 1, sp = 2;
   for (int z = z0; z < z1; ++z) {
     float fr[19], fb[19];
-    ahead.get(rho0, fr, fb);
+    ahead.get(fr, fb);
     const float rR = qR, rB = qB;
     const uint8_t f = qF;
-    plane(z + 1, sp, z + 1 < z1);
+    plane(z + 1, sp);
     __syncthreads();
     if (inside) {
       const uint32_t zs[3] = {(uint32_t)(z == 0 ? nz - 1 : z - 1) * sxy, (uint32_t)z * sxy,
@@ -254,18 +404,6 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
       const uint32_t n = xs[1] + ys[1] + zs[1];
       auto nb3 = [&](int dx, int dy, int dz) -> uint32_t { return xs[dx + 1] + ys[dy + 1] + zs[dz + 1]; };
       if (f == FL_FLUID) {
-        const float rs = rR + rB;
-        const float phf = sphi[s0][ty + 1][c0];
-        const float visc = 1.0f / (0.5f * (1.0f + phf) * a.inv_visc_r + 0.5f * (1.0f - phf) * a.inv_visc_b);
-        const float omega = 1.0f / (visc / CSSQ + 0.5f);
-        const float omm = 1.0f - omega;
-        const float inv = 1.0f / rs;
-        const float u = inv * (fr[1] - fr[2] + fr[7] - fr[8] - fr[9] + fr[10] + fr[11] - fr[12] - fr[13] + fr[14]) +
-                        inv * (fb[1] - fb[2] + fb[7] - fb[8] - fb[9] + fb[10] + fb[11] - fb[12] - fb[13] + fb[14]);
-        const float v = inv * (fr[3] - fr[4] + fr[7] - fr[8] + fr[9] - fr[10] + fr[15] - fr[16] - fr[17] + fr[18]) +
-                        inv * (fb[3] - fb[4] + fb[7] - fb[8] + fb[9] - fb[10] + fb[15] - fb[16] - fb[17] + fb[18]);
-        const float wv = inv * (fr[5] - fr[6] + fr[11] - fr[12] + fr[13] - fr[14] + fr[15] - fr[16] + fr[17] - fr[18]) +
-                         inv * (fb[5] - fb[6] + fb[11] - fb[12] + fb[13] - fb[14] + fb[15] - fb[16] + fb[17] - fb[18]);
         /* 26-point gradient in the D3Q27 order of kernels_fluid_CG.CUF:720-741;
            zero-component terms add nothing and are skipped */
         float px = 0.0f, py = 0.0f, pz = 0.0f;
@@ -279,35 +417,7 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
           if (dy) py = py + A27(l) * (float)dy * ps;
           if (dz) pz = pz + A27(l) * (float)dz * ps;
         }
-        /* BGK towards f^eq(rho_k, u) per colour, :794-800 */
-        const float usq = (0.5f / CSSQ) * (u * u + v * v + wv * wv);
-#pragma unroll
-        for (int l = 0; l < 19; ++l) {
-          const float uv = (1.0f / CSSQ) * (u * (float)EX(l) + v * (float)EY(l) + wv * (float)EZ(l));
-          const float brk = 1.0f + uv + 0.5f * (uv * uv) - usq;
-          fr[l] = fr[l] * omm + rR * W(l) * brk * omega;
-          fb[l] = fb[l] * omm + rB * W(l) * brk * omega;
-        }
-        const float pn2 = px * px + py * py + pz * pz;
-        if (pn2 > 1.e-20f) {
-          /* perturbation (:803-815) and recolouring (:817-837) */
-          const float pn = sqrtf(pn2);
-          const float ipn2 = 1.0f / pn2;
-          const float ipn = 1.0f / pn;
-          const float ac = 9.0f / 4.0f * omega * a.sigma;
-          const float bcoef = a.beta * rR * rB / (rs * rs);
-#pragma unroll
-          for (int l = 0; l < 19; ++l) {
-            const float ed = (float)EX(l) * px + (float)EY(l) * py + (float)EZ(l) * pz;
-            const float tt0 = pn * (W(l) * (ed * ed) * ipn2 - BL(l));
-            const float fd = fr[l] + fb[l] + ac * tt0;
-            const float cph = ed * INV_REC(l) * ipn;
-            const float tt = bcoef * cph;
-            const float feq = rR * W(l) + rB * W(l);
-            fr[l] = fd * rR * inv + tt * feq;
-            fb[l] = fd * rB * inv - tt * feq;
-          }
-        }
+        collide_node<kShifted<S>>(fr, fb, rR, rB, sphi[s0][ty + 1][c0], px, py, pz, a);
       }
       /* push: every node but spheredead sends its 18 moving populations to
          x + e_l and keeps the rest population (flipflop*Pop0_CG, stream*_CG,
@@ -315,14 +425,12 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
          values unchanged (no collision there, :689).  The slots a bounce-back
          link claims are overwritten afterwards by the particle link kernel.
          Stored values of non-fluid nodes are re-stored from the registers
-         they were loaded into: exact for float storage; for 16-bit storage
-         the float round trip (h + s) - s rounds back to h whenever
-         |h + s| < 0.5, which the shift s = w_l rho0 with rho0 <= 1.2
-         (checked at create) and stored deviations |h| < 0.1 satisfy. */
-      stq(dbase(dst, 0, N), n, fr[0], fb[0], W(0) * rho0);
+         they were loaded into, exactly (16-bit values are held as floats
+         without the shift, so the conversion back is exact). */
+      dst.st(0, n, fr[0], fb[0]);
       if (!PART || f != FL_DEAD) {
 #pragma unroll
-        for (int l = 1; l < 19; ++l) stq(dbase(dst, l, N), nb3(EX(l), EY(l), EZ(l)), fr[l], fb[l], W(l) * rho0);
+        for (int l = 1; l < 19; ++l) dst.st(l, nb3(EX(l), EY(l), EZ(l)), fr[l], fb[l]);
       }
     }
     __syncthreads();

@@ -5,9 +5,9 @@ instantiations are compiled separately (1024-lane tiles run under a tighter
 register budget) and agree to an ulp or two, not always bit for bit.  The
 fused kernel reads neighbour phase fields from an LDS ring filled from a halo
 of re-summed nodes; a wrong halo index, a stale ring slot or a wrong periodic
-wrap changes populations by 1e-3 or more, far above the tolerance here.  The fused kernel reads neighbour phase fields from an LDS ring filled
-from a halo of re-summed nodes, so a wrong halo index, a stale ring slot or a
-wrong periodic wrap shows up here as a bit difference.
+wrap changes populations by 1e-3 or more, far above the tolerance here.  Tile
+entries from 10 on keep two planes of loads in flight (a different pipeline
+through the same arithmetic); they are covered the same way.
 
 Also: the autotuner leaves the state untouched and picks a valid tile.
 """
@@ -43,8 +43,11 @@ def valid_tiles(nx):
     return [i for i, (tx, ty, wrap) in enumerate(tile_shapes()) if not wrap or tx == nx]
 
 
-# FP32 rounding of one step, accumulated over 4 steps, for values <= 1
-ULP_TOL = 4e-7
+# FP32 rounding of one step, accumulated over 4 steps, for values <= 1.  The
+# fused kernel forms the collision with per-node constants and 1-ulp
+# reciprocals, the two-kernel step in the reference's operation order, so they
+# differ by a few ulps per step (4e-7 measured), not bit for bit.
+ULP_TOL = 1e-6
 
 
 @pytest.mark.parametrize("shape", [(64, 24, 10), (70, 37, 11), (128, 12, 9), (256, 8, 5)])
@@ -70,7 +73,8 @@ def test_fused_equals_twopass_with_static_particles():
     kw = dict(with_particles=1, n_particles=4, lrotate=1, rmass=0.0, max_vel=1.0, wetting=WET_NEUTRAL)
     two = run((n, n, n), 6, KERNEL_TWOPASS, pos=pos, cfg_kw=kw)
     fluid = (two[2] == 1)[..., None].repeat(19, -1)
-    for t in valid_tiles(n)[:3]:
+    tiles = valid_tiles(n)
+    for t in tiles[:3] + [t for t in tiles if t >= 10][:2]:
         got = run((n, n, n), 6, KERNEL_FUSED, t, 7, pos=pos, cfg_kw=kw)
         assert np.array_equal(got[2], two[2])
         for c in (0, 1):
