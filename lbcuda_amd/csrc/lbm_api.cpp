@@ -288,7 +288,7 @@ int lbm_destroy(lbm_ctx* c) {
   dfree(c->d_status);
   dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
   dfree(c->d_claim); dfree(c->d_tmpl); dfree(c->d_ball); dfree(c->d_ring); dfree(c->d_tlist);
-  dfree(c->d_nbr); dfree(c->d_nnbr); dfree(c->d_contrib); dfree(c->d_closure);
+  dfree(c->d_nbr); dfree(c->d_nnbr); dfree(c->d_contrib); dfree(c->d_smask); dfree(c->d_closure);
   for (int e = 0; e < 2 * c->ev_cap; ++e) (void)hipEventDestroy(c->ev_pairs[e]);
   delete[] c->ev_pairs;
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -372,11 +372,12 @@ int lbm_set_particles(lbm_ctx* c, int32_t n, const double* pos) {
     c->d_myf = c->d_myf2 = c->d_myt = c->d_myt2 = nullptr;
     int rc = 0;
 
-    dfree(c->d_nbr); dfree(c->d_nnbr); dfree(c->d_contrib);
-    c->d_nbr = nullptr; c->d_nnbr = nullptr; c->d_contrib = nullptr;
+    dfree(c->d_nbr); dfree(c->d_nnbr); dfree(c->d_contrib); dfree(c->d_smask);
+    c->d_nbr = nullptr; c->d_nnbr = nullptr; c->d_contrib = nullptr; c->d_smask = nullptr;
     rc |= dalloc(&c->d_nbr, (size_t)lbm_particles_max_neighbours() * n);
     rc |= dalloc(&c->d_contrib, (size_t)6 * lbm_particles_max_neighbours() * n);
     rc |= dalloc(&c->d_nnbr, (size_t)n);
+    rc |= dalloc(&c->d_smask, (size_t)c->ntlist * n);
     for (int b = 0; b < 2; ++b) {
       rc |= dalloc(&c->d_pos[b], (size_t)13 * n);
       rc |= dalloc(&c->d_force[b], (size_t)12 * n);

@@ -269,6 +269,7 @@ struct lbm_ctx {
   int* d_nbr;             /* (32, np) neighbour lists of the link kernel        */
   int* d_nnbr;            /* (np)                                               */
   double* d_contrib;      /* (6, 32, np) link momentum shares between particles */
+  uint32_t* d_smask;      /* (ntlist, np) link masks of owned surface nodes      */
   int8_t* d_tmpl;         /* (4, ntmpl) template offsets + value               */
   int ntmpl;
   int8_t* d_ring;         /* (4, nring) D3Q27 ring of the template's surface */

@@ -63,6 +63,13 @@ struct PartArgs {
   int* nnbr;               /* (np) number found                         */
   double* contrib;         /* (6, MAXNB, np) link momentum a particle's
                               surface gives to each of its neighbours  */
+  uint32_t* smask;         /* (ntlist, np) for each surface template node
+                              the particle owns: bit l-1 set if the D3Q19
+                              neighbour l is fluid, bit 31 set; 0 for nodes
+                              it does not own.  Written by the surface
+                              pass before the collision, read by the link
+                              pass after it (node types do not change in
+                              between)                                  */
   int rmax;
   const int8_t* ball;      /* (4, nball), [3] = 1 if r^2 < rmax^2 */
   int nball;
@@ -413,83 +420,101 @@ __device__ __forceinline__ void k_fill_release_body(const PartArgs& a, int p) {
 }
 
 /* ---------------- wetting of surface nodes -------------------------------- */
-/* densities of surface node t of the list of particle p (centre c), if p owns
- * it: Janus or neutral wetting */
+/* surface node t of the template list of particle p (centre c): if p owns
+ * it, its D3Q19 fluid-neighbour mask for the link pass and, unless wetting is
+ * off, its Janus or neutral wetting densities */
 template <class S>
-__device__ __forceinline__ void wet_node(const PartArgs& a, int p, const int c[3], int t) {
-  {
-    int gx, gy, gz;
-    size_t n = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
-    if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) return;
-    float rr, rb;
-    if (a.wetting == LBM_WET_JANUS) {
-      /* compute_virtual_rho, PART:299-352: sign of r.(q x q*) */
-      double r[3] = {(double)(gx + 1) - a.pos_cur[13 * p], (double)(gy + 1) - a.pos_cur[13 * p + 1],
-                     (double)(gz + 1) - a.pos_cur[13 * p + 2]};
-      r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
-      r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
-      r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
-      const double q0 = a.pos_cur[13 * p + 9], q1 = a.pos_cur[13 * p + 10], q2 = a.pos_cur[13 * p + 11],
-                   q3 = a.pos_cur[13 * p + 12];
-      /* body x axis in the lab frame: first column of the rotation matrix */
-      const double ax = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
-      const double ay = 2.0 * (q1 * q2 + q0 * q3);
-      const double az = 2.0 * (q1 * q3 - q0 * q2);
-      const double ac = ax * r[0] + ay * r[1] + az * r[2];
-      if (ac < 0.0) { rr = 0.0f; rb = 1.0f; } else { rr = 1.0f; rb = 0.0f; }
-    } else {
-      /* compute_densities_wall + compute_onebelt_density_2fl_weights, PART:267-297, 864-894;
-         the 26 neighbours' types and densities are loaded in one batch */
-      uint8_t f27[27];
-      float r27[27], b27[27];
+__device__ __forceinline__ void surface_node(const PartArgs& a, int p, const int c[3], int t) {
+  int gx, gy, gz;
+  size_t n = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
+  uint32_t* mslot = a.smask + (size_t)p * a.ntlist + t;
+  if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) {
+    *mslot = 0u;
+    return;
+  }
+  /* the 26 neighbours' types (and densities for neutral wetting) in one batch */
+  uint8_t f27[27];
+  float r27[27], b27[27];
+  if (a.wetting == LBM_WET_NEUTRAL) {
 #pragma unroll
-      for (int l = 1; l < 27; ++l) {
-        const size_t m = wrap_node(a, gx + EX27(l), gy + EY27(l), gz + EZ27(l));
-        f27[l] = a.fl_cur[m];
-        r27[l] = a.rho_r[m];
-        b27[l] = a.rho_b[m];
-      }
-      float Rs = 0.0f, Bs = 0.0f, Ds = 0.0f;
+    for (int l = 1; l < 27; ++l) {
+      const size_t m = wrap_node(a, gx + EX27(l), gy + EY27(l), gz + EZ27(l));
+      f27[l] = a.fl_cur[m];
+      r27[l] = a.rho_r[m];
+      b27[l] = a.rho_b[m];
+    }
+  } else {
 #pragma unroll
-      for (int l = 1; l < 27; ++l) {
-        if (f27[l] == FL_FLUID) {
-          Rs = Rs + PD27(l) * r27[l];
-          Bs = Bs + PD27(l) * b27[l];
-          Ds = Ds + PD27(l);
-        }
-      }
-      if (Ds != 0.0f) {
-        rr = Rs / Ds;
-        rb = Bs / Ds;
-      } else {
-        float R2 = 0.0f, B2 = 0.0f;
-        int D = 0;
-        for (int l = 0; l < 98; ++l) {
-          size_t m = wrap_node(a, gx + C_BELT2[l][0], gy + C_BELT2[l][1], gz + C_BELT2[l][2]);
-          if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
-            float mr, mb;
-            belt_rho<S>(a, m, &mr, &mb);
-            R2 = R2 + mr;
-            B2 = B2 + mb;
-            D = D + 1;
-          }
-        }
-        if (D == 0) return;   /* fix_onebelt_density_twofluids is empty */
-        rr = R2 / (float)D;
-        rb = B2 / (float)D;
+    for (int l = 1; l < 27; ++l) {
+      f27[l] = a.fl_cur[wrap_node(a, gx + EX27(l), gy + EY27(l), gz + EZ27(l))];
+      r27[l] = 0.0f;
+      b27[l] = 0.0f;
+    }
+  }
+  uint32_t mask = 1u << 31;
+#pragma unroll
+  for (int l = 1; l < 19; ++l)
+    if (f27[l] == FL_FLUID) mask |= 1u << (l - 1);
+  *mslot = mask;
+  if (a.wetting == LBM_WET_NONE) return;
+  float rr, rb;
+  if (a.wetting == LBM_WET_JANUS) {
+    /* compute_virtual_rho, PART:299-352: sign of r.(q x q*) */
+    double r[3] = {(double)(gx + 1) - a.pos_cur[13 * p], (double)(gy + 1) - a.pos_cur[13 * p + 1],
+                   (double)(gz + 1) - a.pos_cur[13 * p + 2]};
+    r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
+    r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
+    r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
+    const double q0 = a.pos_cur[13 * p + 9], q1 = a.pos_cur[13 * p + 10], q2 = a.pos_cur[13 * p + 11],
+                 q3 = a.pos_cur[13 * p + 12];
+    /* body x axis in the lab frame: first column of the rotation matrix */
+    const double ax = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    const double ay = 2.0 * (q1 * q2 + q0 * q3);
+    const double az = 2.0 * (q1 * q3 - q0 * q2);
+    const double ac = ax * r[0] + ay * r[1] + az * r[2];
+    if (ac < 0.0) { rr = 0.0f; rb = 1.0f; } else { rr = 1.0f; rb = 0.0f; }
+  } else {
+    /* compute_densities_wall + compute_onebelt_density_2fl_weights, PART:267-297, 864-894 */
+    float Rs = 0.0f, Bs = 0.0f, Ds = 0.0f;
+#pragma unroll
+    for (int l = 1; l < 27; ++l) {
+      if (f27[l] == FL_FLUID) {
+        Rs = Rs + PD27(l) * r27[l];
+        Bs = Bs + PD27(l) * b27[l];
+        Ds = Ds + PD27(l);
       }
     }
-    a.rho_r[n] = rr;
-    a.rho_b[n] = rb;
-    a.phi[n] = (rr - rb) / (rr + rb);
+    if (Ds != 0.0f) {
+      rr = Rs / Ds;
+      rb = Bs / Ds;
+    } else {
+      float R2 = 0.0f, B2 = 0.0f;
+      int D = 0;
+      for (int l = 0; l < 98; ++l) {
+        size_t m = wrap_node(a, gx + C_BELT2[l][0], gy + C_BELT2[l][1], gz + C_BELT2[l][2]);
+        if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
+          float mr, mb;
+          belt_rho<S>(a, m, &mr, &mb);
+          R2 = R2 + mr;
+          B2 = B2 + mb;
+          D = D + 1;
+        }
+      }
+      if (D == 0) return;   /* fix_onebelt_density_twofluids is empty */
+      rr = R2 / (float)D;
+      rb = B2 / (float)D;
+    }
   }
+  a.rho_r[n] = rr;
+  a.rho_b[n] = rb;
+  a.phi[n] = (rr - rb) / (rr + rb);
 }
 
 template <class S>
 __device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
-  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntlist; t += TPB * gridDim.y) wet_node<S>(a, p, c, t);
+  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntlist; t += TPB * gridDim.y) surface_node<S>(a, p, c, t);
 }
 
 /* ---------------- densities around each particle (single-pass step) -----
@@ -526,19 +551,19 @@ __device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], in
   a.phi[n] = (rr - rb) / (rr + rb);
 }
 
-/* shell densities and wetting in one pass, one workgroup per particle (runs
- * after make_fluid).  Every fluid D3Q27 neighbour of a surface node p owns is
- * in p's ring, so the wetting after the barrier reads only densities this
- * workgroup wrote (or make_fluid wrote before the launch). */
+/* shell densities, wetting and the link masks of the surface nodes in one
+ * pass, one workgroup per particle (runs after make_fluid).  Every fluid
+ * D3Q27 neighbour of a surface node p owns is in p's ring, so the wetting
+ * after the barrier reads only densities this workgroup wrote (or make_fluid
+ * wrote before the launch). */
 template <class S>
 __device__ __forceinline__ void k_shellwet_body(const PartArgs& a, int p) {
   int c[3];
   centre(a.pos_cur, p, c);
   for (int t = threadIdx.x; t < a.ntmpl; t += TPB) shell_zero(a, p, c, t);
   for (int t = threadIdx.x; t < a.nring; t += TPB) shell_ring<S>(a, c, t);
-  if (a.wetting == LBM_WET_NONE) return;
   __syncthreads();
-  for (int t = threadIdx.x; t < a.ntlist; t += TPB) wet_node<S>(a, p, c, t);
+  for (int t = threadIdx.x; t < a.ntlist; t += TPB) surface_node<S>(a, p, c, t);
 }
 
 /* ---------------- bounce-back on surface links ---------------------------
@@ -572,32 +597,32 @@ __device__ __forceinline__ int ball_r2(const PartArgs& a, const int c[3], int gx
   return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
 }
 
-/* what the link pass reads around one surface node s after the collide: the
- * types of its 18 D3Q19 neighbours (one batch), then for each link, i.e.
- * each opposite pair (lo, hi) with exactly one fluid neighbour ii = s +
- * e_lin, the densities of ii and the value ii pushed into s, dst(s, lout)
- * (second batch; directions without a link are not read) */
+/* what the link pass reads around one surface node s after the collide, for
+ * each link, i.e. each opposite pair il = (lo, hi) = (2 il - 1, 2 il) with
+ * exactly one fluid neighbour ii = s + e_lin (from the mask the surface pass
+ * recorded): the densities of ii and the value ii pushed into s,
+ * dst(s, lout).  All of them are issued in one batch. */
 template <class S>
-struct SurfaceNode {
-  size_t nb[19];
-  uint8_t f[19];
-  float rr[19], rb[19], pr[19], pb[19];
-  __device__ __forceinline__ void load(const PartArgs& a, size_t s, int gx, int gy, int gz) {
-#pragma unroll
-    for (int l = 1; l < 19; ++l) {
-      nb[l] = wrap_node(a, gx + EX(l), gy + EY(l), gz + EZ(l));
-      f[l] = a.fl_cur[nb[l]];
-    }
+struct LinkData {
+  uint32_t mask;           /* D3Q19 fluid-neighbour mask of s */
+  uint32_t nb[10];         /* ii of link il                   */
+  float rr[10], rb[10], pr[10], pb[10];
+  __device__ static bool is_link(uint32_t m, int il) {
+    return ((m >> (2 * il - 2)) & 1u) != ((m >> (2 * il - 1)) & 1u);
+  }
+  __device__ static bool lo_is_fluid(uint32_t m, int il) { return (m >> (2 * il - 2)) & 1u; }
+  __device__ __forceinline__ void load(const PartArgs& a, size_t s, int gx, int gy, int gz, uint32_t m) {
+    mask = m;
 #pragma unroll
     for (int il = 1; il <= 9; ++il) {
-      const int lo = 2 * il - 1, hi = 2 * il;
-      const bool fi = f[lo] == FL_FLUID, fo = f[hi] == FL_FLUID;
-      if (fi == fo) continue;
-      const int lin = fi ? lo : hi;
-      const int lout = fi ? hi : lo;
-      rr[lin] = a.rho_r[nb[lin]];
-      rb[lin] = a.rho_b[nb[lin]];
-      ld_pair<S>(a.pop_dst, a.N, lout, s, a.rho0, pr[lout], pb[lout]);
+      if (!is_link(m, il)) continue;
+      const bool fi = lo_is_fluid(m, il);
+      const int lin = fi ? 2 * il - 1 : 2 * il;
+      const int lout = fi ? 2 * il : 2 * il - 1;
+      nb[il] = (uint32_t)wrap_node(a, gx + EX(lin), gy + EY(lin), gz + EZ(lin));
+      rr[il] = a.rho_r[nb[il]];
+      rb[il] = a.rho_b[nb[il]];
+      ld_pair<S>(a.pop_dst, a.N, lout, s, a.rho0, pr[il], pb[il]);
     }
   }
 };
@@ -610,14 +635,16 @@ struct NbrCache {
   double o[MAXNB][3];
 };
 
-/* wall velocity e_lout . v_b of particle k (cache slot) at the link of s */
-__device__ __forceinline__ void link_velocity(const PartArgs& a, const NbrCache& nc, int k, int gx, int gy, int gz,
+/* wall velocity v_b of particle k (cache slot) at the link lin of s, with
+ * the particle's lever R r/|r| to s (rv, formed once per node): v + omega x
+ * (rv + e_lin / 2) when rotating (PART:1900-1927, 1954-1962); rt is the
+ * torque arm rv + e_lin / 2 */
+__device__ __forceinline__ void link_velocity(const PartArgs& a, const NbrCache& nc, int k, const double rv[3],
                                               int lin, double v[3], double rt[3]) {
   v[0] = nc.v[k][0]; v[1] = nc.v[k][1]; v[2] = nc.v[k][2];
   rt[0] = rt[1] = rt[2] = 0.0;
   if (a.lrotate) {
-    double rv[3], cr[3];
-    lever(a, a.pos_cur, nc.q[k], gx, gy, gz, true, rv);
+    double cr[3];
     rt[0] = rv[0] + 0.5 * EX(lin); rt[1] = rv[1] + 0.5 * EY(lin); rt[2] = rv[2] + 0.5 * EZ(lin);
     const double ot[3] = {nc.o[k][0], nc.o[k][1], nc.o[k][2]};
     cross3(ot, rt, cr);
@@ -625,34 +652,34 @@ __device__ __forceinline__ void link_velocity(const PartArgs& a, const NbrCache&
   }
 }
 
-/* momentum given to particle k by the links of surface node s
- * (addForce at :1964-1972 / :2049-2057, torque :1999-2001) */
+/* momentum given to particle k by link il of s (addForce at :1964-1972 /
+ * :2049-2057, torque :1999-2001), from the pushed values before the
+ * bounce-back */
 template <class S>
-__device__ __forceinline__ void link_force(const PartArgs& a, const NbrCache& nc, int k, const SurfaceNode<S>& sn,
-                                           int gx, int gy, int gz, double acc[6]) {
-#pragma unroll
-  for (int il = 1; il <= 9; ++il) {
-    const int lo = 2 * il - 1, hi = 2 * il;
-    const bool fi = sn.f[lo] == FL_FLUID, fo = sn.f[hi] == FL_FLUID;
-    if (fi == fo) continue;
-    const int lin = fi ? lo : hi;      /* from s toward the fluid node   */
-    const int lout = fi ? hi : lo;     /* from the fluid node toward s   */
-    double v[3], rt[3];
-    link_velocity(a, nc, k, gx, gy, gz, lin, v, rt);
-    const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
-    const float wr = W(lout) * BUZ * sn.rr[lin] * ev;
-    const float wb = W(lout) * BUZ * sn.rb[lin] * ev;
-    const double f2p = (double)((2.0f * sn.pr[lout] - wr) + (2.0f * sn.pb[lout] - wb));
-    const double ft[3] = {f2p * EX(lout), f2p * EY(lout), f2p * EZ(lout)};
-    acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
-    if (a.lrotate) {
-      double tq[3];
-      cross3(rt, ft, tq);
-      acc[3] += tq[0] * abs(EX(lout));
-      acc[4] += tq[1] * abs(EY(lout));
-      acc[5] += tq[2] * abs(EZ(lout));
-    }
+__device__ __forceinline__ void link_force_one(const PartArgs& a, const LinkData<S>& ld, int il, const double v[3],
+                                               const double rt[3], double acc[6]) {
+  const bool fi = LinkData<S>::lo_is_fluid(ld.mask, il);
+  const int lout = fi ? 2 * il : 2 * il - 1;
+  const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
+  const float wr = W(lout) * BUZ * ld.rr[il] * ev;
+  const float wb = W(lout) * BUZ * ld.rb[il] * ev;
+  const double f2p = (double)((2.0f * ld.pr[il] - wr) + (2.0f * ld.pb[il] - wb));
+  const double ft[3] = {f2p * EX(lout), f2p * EY(lout), f2p * EZ(lout)};
+  acc[0] += ft[0]; acc[1] += ft[1]; acc[2] += ft[2];
+  if (a.lrotate) {
+    double tq[3];
+    cross3(rt, ft, tq);
+    acc[3] += tq[0] * abs(EX(lout));
+    acc[4] += tq[1] * abs(EY(lout));
+    acc[5] += tq[2] * abs(EZ(lout));
   }
+}
+
+/* lever of cache slot k to node (gx, gy, gz), when rotating */
+__device__ __forceinline__ void slot_lever(const PartArgs& a, const NbrCache& nc, int k, int gx, int gy, int gz,
+                                           double rv[3]) {
+  rv[0] = rv[1] = rv[2] = 0.0;
+  if (a.lrotate) lever(a, a.pos_cur, nc.q[k], gx, gy, gz, true, rv);
 }
 
 template <class S>
@@ -681,37 +708,53 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
   /* a full neighbour list (LBM_ERR_LIST_OVERFLOW raised by k_neigh) may miss
      the particle itself: leave its link sums untouched then */
   if (kp < 0) return;
+  const uint32_t* msk = a.smask + (size_t)p * a.ntlist;
   double own[6] = {0, 0, 0, 0, 0, 0};
   bool shared_node = false;   /* some surface node of this thread is covered by another particle */
   for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
+    const uint32_t m = msk[t];
+    if (!(m >> 31)) continue;
     int gx, gy, gz;
     const size_t s = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
-    if (a.fl_cur[s] != FL_LIST || (a.key_cur[s] >> 3) != (uint32_t)p) continue;
-    SurfaceNode<S> sn;
-    sn.load(a, s, gx, gy, gz);
+    LinkData<S> ld;
+    ld.load(a, s, gx, gy, gz, m);
     unsigned cover = 0;   /* cache slots of the particles whose ball holds s */
     for (int k = 0; k < nnb; ++k)
       if (ball_r2(a, nc.c[k], gx, gy, gz) < r2max) cover |= 1u << k;
     if (cover & ~(1u << kp)) shared_node = true;
+    /* bounce-back: dst(ii, lin) = dst(s, lout) minus the moving-wall term of
+       every covering particle, in ascending order; the owner's link momentum
+       on the way */
+    float fR[10], fB[10];
 #pragma unroll
     for (int il = 1; il <= 9; ++il) {
-      const int lo = 2 * il - 1, hi = 2 * il;
-      const bool fi = sn.f[lo] == FL_FLUID, fo = sn.f[hi] == FL_FLUID;
-      if (fi == fo) continue;
-      const int lin = fi ? lo : hi;
-      const int lout = fi ? hi : lo;
-      float fR = sn.pr[lout], fB = sn.pb[lout];
-      for (int k = 0; k < nnb; ++k) {
-        if (!(cover >> k & 1u)) continue;
-        double v[3], rt[3];
-        link_velocity(a, nc, k, gx, gy, gz, lin, v, rt);
-        const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
-        fR = fR - W(lout) * BUZ * sn.rr[lin] * ev;
-        fB = fB - W(lout) * BUZ * sn.rb[lin] * ev;
-      }
-      st_pair<S>(a.pop_dst, a.N, lin, sn.nb[lin], fR, fB, a.rho0);
+      fR[il] = ld.pr[il];
+      fB[il] = ld.pb[il];
     }
-    link_force<S>(a, nc, kp, sn, gx, gy, gz, own);
+    for (int k = 0; k < nnb; ++k) {
+      if (!(cover >> k & 1u)) continue;
+      double rv[3];
+      slot_lever(a, nc, k, gx, gy, gz, rv);
+#pragma unroll
+      for (int il = 1; il <= 9; ++il) {
+        if (!LinkData<S>::is_link(m, il)) continue;
+        const bool fi = LinkData<S>::lo_is_fluid(m, il);
+        const int lin = fi ? 2 * il - 1 : 2 * il;
+        const int lout = fi ? 2 * il : 2 * il - 1;
+        double v[3], rt[3];
+        link_velocity(a, nc, k, rv, lin, v, rt);
+        const float ev = (float)(EX(lout) * v[0] + EY(lout) * v[1] + EZ(lout) * v[2]);
+        fR[il] = fR[il] - W(lout) * BUZ * ld.rr[il] * ev;
+        fB[il] = fB[il] - W(lout) * BUZ * ld.rb[il] * ev;
+        if (k == kp) link_force_one<S>(a, ld, il, v, rt, own);
+      }
+    }
+#pragma unroll
+    for (int il = 1; il <= 9; ++il) {
+      if (!LinkData<S>::is_link(m, il)) continue;
+      const int lin = LinkData<S>::lo_is_fluid(m, il) ? 2 * il - 1 : 2 * il;
+      st_pair<S>(a.pop_dst, a.N, lin, ld.nb[il], fR[il], fB[il], a.rho0);
+    }
   }
   block_reduce6(own);
   if (threadIdx.x == 0) {
@@ -724,17 +767,29 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
   for (int k = 0; k < nnb; ++k) {
     if (k == kp) continue;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (any_shared)
+    if (any_shared) {
       for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
+        const uint32_t m = msk[t];
+        if (!(m >> 31)) continue;
         int gx, gy, gz;
-        const size_t s = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
-        if (a.fl_cur[s] != FL_LIST || (a.key_cur[s] >> 3) != (uint32_t)p) continue;
+        const size_t s =
+            node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
         if (ball_r2(a, nc.c[k], gx, gy, gz) >= r2max) continue;
-        SurfaceNode<S> sn;
-        sn.load(a, s, gx, gy, gz);
-        link_force<S>(a, nc, k, sn, gx, gy, gz, acc);
+        LinkData<S> ld;
+        ld.load(a, s, gx, gy, gz, m);
+        double rv[3];
+        slot_lever(a, nc, k, gx, gy, gz, rv);
+#pragma unroll
+        for (int il = 1; il <= 9; ++il) {
+          if (!LinkData<S>::is_link(m, il)) continue;
+          const int lin = LinkData<S>::lo_is_fluid(m, il) ? 2 * il - 1 : 2 * il;
+          double v[3], rt[3];
+          link_velocity(a, nc, k, rv, lin, v, rt);
+          link_force_one<S>(a, ld, il, v, rt, acc);
+        }
       }
-    block_reduce6(acc);
+      block_reduce6(acc);
+    }
     if (threadIdx.x == 0)
       for (int d = 0; d < 6; ++d) a.contrib[6 * ((size_t)MAXNB * p + k) + d] = acc[d];
   }
@@ -1087,7 +1142,7 @@ PartArgs make_pargs(lbm_ctx* c) {
   a.tmpl = c->d_tmpl; a.ntmpl = c->ntmpl;
   a.ring = c->d_ring; a.nring = c->nring;
   a.tlist = c->d_tlist; a.ntlist = c->ntlist;
-  a.nbr = c->d_nbr; a.nnbr = c->d_nnbr; a.contrib = c->d_contrib;
+  a.nbr = c->d_nbr; a.nnbr = c->d_nnbr; a.contrib = c->d_contrib; a.smask = c->d_smask;
   a.rmax = c->rmax;
   a.ball = c->d_ball; a.nball = c->nball;
   a.rho_r = c->rho_r; a.rho_b = c->rho_b; a.phi = c->phi;
@@ -1166,7 +1221,6 @@ int lbm_particles_fill(lbm_ctx* c) {
 }
 
 int lbm_particles_wetting(lbm_ctx* c) {
-  if (c->cfg.wetting == LBM_WET_NONE) return 0;
   PartArgs a = make_pargs(c);
   const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
   LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_wetting<S>, g, dim3(TPB), 0, c->stream, a))
