@@ -251,7 +251,9 @@ __device__ __forceinline__ void k_resolve_body(const PartArgs& a, int p) {
   }
 }
 
-/* ---------------- erase: fluid -> solid (del_fluid, PART:383-561) -------- */
+/* ---------------- erase: fluid -> solid (del_fluid, PART:383-561) --------
+ * The same pass over the old ball claims the nodes that turn fluid for
+ * make_fluid (k_fill_apply). */
 template <class S>
 __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
@@ -261,8 +263,11 @@ __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
   for (int t = threadIdx.x; t < a.nball; t += TPB) {
     int gx, gy, gz;
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
-    const uint8_t fnew = a.fl_cur[n];
-    if (a.fl_old[n] != FL_FLUID || !(fnew == FL_LIST || fnew == FL_DEAD)) continue;
+    const uint8_t fnew = a.fl_cur[n], fold = a.fl_old[n];
+    /* a node that turns fluid is claimed for make_fluid: the highest index
+       among the moving particles whose old ball holds it writes it */
+    if ((fold == FL_LIST || fold == FL_DEAD) && fnew == FL_FLUID) atomicMax(&a.claim[n], p);
+    if (fold != FL_FLUID || !(fnew == FL_LIST || fnew == FL_DEAD)) continue;
     /* rhoR_d/rhoB_d still hold the previous step's densities here (del_fluid
        runs before init_rho*_CG, main.CUF:130-144); the single-pass step does
        not store them, so they are summed from the previous level */
@@ -304,16 +309,6 @@ __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
 __device__ bool fill_candidate(const PartArgs& a, size_t n) {
   const uint8_t fo = a.fl_old[n];
   return (fo == FL_LIST || fo == FL_DEAD) && a.fl_cur[n] == FL_FLUID;
-}
-
-__device__ __forceinline__ void k_fill_claim_body(const PartArgs& a, int p) {
-  if (!moved(a, p)) return;
-  int co[3];
-  centre(a.pos_old, p, co);
-  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.nball; t += TPB * gridDim.y) {
-    size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2]);
-    if (fill_candidate(a, n)) atomicMax(&a.claim[n], p);
-  }
 }
 
 template <class S>
@@ -551,6 +546,8 @@ __device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], in
   a.phi[n] = (rr - rb) / (rr + rb);
 }
 
+__device__ __forceinline__ void k_neigh_body(const PartArgs& a, int i);
+
 /* shell densities, wetting and the link masks of the surface nodes in one
  * pass, one workgroup per particle (runs after make_fluid).  Every fluid
  * D3Q27 neighbour of a surface node p owns is in p's ring, so the wetting
@@ -558,6 +555,10 @@ __device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], in
  * wrote before the launch). */
 template <class S>
 __device__ __forceinline__ void k_shellwet_body(const PartArgs& a, int p) {
+  /* the neighbour list the link pass uses (positions do not change before
+     it) and the release of make_fluid's claims (every particle's fill is done) */
+  k_neigh_body(a, p);
+  k_fill_release_body(a, p);
   int c[3];
   centre(a.pos_cur, p, c);
   for (int t = threadIdx.x; t < a.ntmpl; t += TPB) shell_zero(a, p, c, t);
@@ -581,7 +582,7 @@ __device__ __forceinline__ void k_shellwet_body(const PartArgs& a, int p) {
  *    summed in FP64 over the workgroup into myf/myt; the share of another
  *    particle q (two particles in contact) is summed the same way into
  *    contrib(slot of q) and added to q's force, in ascending owner order, by
- *    k_integrate.  The sums never depend on scheduling.
+ *    the rigid-body update.  The sums never depend on scheduling.
  */
 
 /* integer minimum-image squared distance of node (gx,gy,gz) (0-based) from
@@ -1093,7 +1094,6 @@ __device__ __forceinline__ PartArgs batch_args(const PartArgs* __restrict__ args
 LBM_PART_KERNEL(k_unstamp)
 LBM_PART_KERNEL(k_stamp)
 LBM_PART_KERNEL(k_resolve)
-LBM_PART_KERNEL(k_fill_claim)
 LBM_PART_KERNEL(k_fill_release)
 LBM_PART_KERNEL(k_neigh)
 LBM_PART_KERNEL_S(k_erase)
@@ -1102,24 +1102,21 @@ LBM_PART_KERNEL_S(k_wetting)
 LBM_PART_KERNEL_S(k_shellwet)
 LBM_PART_KERNEL_S(k_links)
 
-__global__ __launch_bounds__(TPB) void k_pairs(PartArgs a, PairParams q) { k_pairs_body(a, blockIdx.x, q); }
-__global__ __launch_bounds__(TPB) void k_pairs_batch(const PartArgs* __restrict__ args,
-                                                     const PairParams* __restrict__ qs,
-                                                     const int2* __restrict__ items, BatchStep b) {
+/* pair forces of particle i, then (thread 0) its rigid-body update: the
+ * update needs only i's own sums and the link shares k_links wrote */
+__device__ __forceinline__ void k_forces_body(const PartArgs& a, int i, const PairParams& pp, const IntParams& ip) {
+  k_pairs_body(a, i, pp);
+  if (threadIdx.x == 0) k_integrate_body(a, i, ip);
+}
+__global__ __launch_bounds__(TPB) void k_forces(PartArgs a, PairParams pp, IntParams ip) {
+  k_forces_body(a, blockIdx.x, pp, ip);
+}
+__global__ __launch_bounds__(TPB) void k_forces_batch(const PartArgs* __restrict__ args,
+                                                      const PairParams* __restrict__ pps,
+                                                      const IntParams* __restrict__ ips,
+                                                      const int2* __restrict__ items, BatchStep b) {
   const int2 it = items[blockIdx.x];
-  k_pairs_body(batch_args(args, it.x, b), it.y, qs[it.x]);
-}
-__global__ __launch_bounds__(TPB) void k_integrate(PartArgs a, IntParams q) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.np) k_integrate_body(a, i, q);
-}
-__global__ __launch_bounds__(TPB) void k_integrate_batch(const PartArgs* __restrict__ args,
-                                                         const IntParams* __restrict__ qs,
-                                                         const int2* __restrict__ items, int nitems, BatchStep b) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nitems) return;
-  const int2 it = items[g];
-  k_integrate_body(batch_args(args, it.x, b), it.y, qs[it.x]);
+  k_forces_body(batch_args(args, it.x, b), it.y, pps[it.x], ips[it.x]);
 }
 
 PartArgs make_pargs(lbm_ctx* c) {
@@ -1211,12 +1208,13 @@ int lbm_particles_erase(lbm_ctx* c) {
   return last_err();
 }
 
+/* make_fluid; the nodes were claimed by the erase pass, and the claims are
+ * released here or, in the single-pass step, by the shell pass */
 int lbm_particles_fill(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
   const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
-  hipLaunchKernelGGL(k_fill_claim, g, dim3(TPB), 0, c->stream, a);
   LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_fill_apply<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
-  hipLaunchKernelGGL(k_fill_release, g, dim3(TPB), 0, c->stream, a);
+  if (c->variant != LBM_KERNEL_FUSED) hipLaunchKernelGGL(k_fill_release, g, dim3(TPB), 0, c->stream, a);
   return last_err();
 }
 
@@ -1224,6 +1222,7 @@ int lbm_particles_wetting(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
   const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
   LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_wetting<S>, g, dim3(TPB), 0, c->stream, a))
+  hipLaunchKernelGGL(k_neigh, dim3(c->np), dim3(TPB), 0, c->stream, a);
   return last_err();
 }
 
@@ -1235,7 +1234,6 @@ int lbm_particles_shell_wetting(lbm_ctx* c) {
 
 int lbm_particles_bounce(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
-  hipLaunchKernelGGL(k_neigh, dim3(c->np), dim3(TPB), 0, c->stream, a);
   LBM_STORAGE_SWITCH(c->storage,
     hipLaunchKernelGGL(k_links<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
   return last_err();
@@ -1278,9 +1276,7 @@ int lbm_particles_integrate(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
   const PairParams pp = make_pair_params(c);
   const IntParams ip = make_int_params(c);
-  unsigned nb = (unsigned)((c->np + TPB - 1) / TPB);
-  hipLaunchKernelGGL(k_pairs, dim3(c->np), dim3(TPB), 0, c->stream, a, pp);
-  hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(TPB), 0, c->stream, a, ip);
+  hipLaunchKernelGGL(k_forces, dim3(c->np), dim3(TPB), 0, c->stream, a, pp, ip);
   return last_err();
 }
 
@@ -1364,9 +1360,7 @@ int lbm_part_batch_pre(lbm_part_batch* b, int parity, long step, hipStream_t st)
   hipLaunchKernelGGL(k_resolve_batch, gc, t, 0, st, A, b->d_items, bs);
   LBM_STORAGE_SWITCH(b->storage,
     hipLaunchKernelGGL(k_erase_batch<S>, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_fill_claim_batch, gc, t, 0, st, A, b->d_items, bs);
     hipLaunchKernelGGL(k_fill_apply_batch<S>, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_fill_release_batch, gc, t, 0, st, A, b->d_items, bs);
     hipLaunchKernelGGL(k_shellwet_batch<S>, g, t, 0, st, A, b->d_items, bs))
   return last_err();
 }
@@ -1379,10 +1373,7 @@ int lbm_part_batch_post(lbm_part_batch* b, int parity, long step, hipStream_t st
   bs.step = (int)step;
   bs.lag_from_pops = step > 1 ? 1 : 0;
   const dim3 g((unsigned)b->nitems), t(TPB);
-  hipLaunchKernelGGL(k_neigh_batch, g, t, 0, st, A, b->d_items, bs);
   LBM_STORAGE_SWITCH(b->storage, hipLaunchKernelGGL(k_links_batch<S>, g, t, 0, st, A, b->d_items, bs))
-  hipLaunchKernelGGL(k_pairs_batch, g, t, 0, st, A, b->d_pp, b->d_items, bs);
-  hipLaunchKernelGGL(k_integrate_batch, dim3((unsigned)((b->nitems + TPB - 1) / TPB)), t, 0, st, A, b->d_ip,
-                     b->d_items, b->nitems, bs);
+  hipLaunchKernelGGL(k_forces_batch, g, t, 0, st, A, b->d_pp, b->d_ip, b->d_items, bs);
   return last_err();
 }
