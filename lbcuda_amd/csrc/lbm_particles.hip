@@ -423,7 +423,9 @@ __device__ __forceinline__ void surface_node(const PartArgs& a, int p, const int
   int gx, gy, gz;
   size_t n = node_at(a, c[0], c[1], c[2], a.tlist[4 * t], a.tlist[4 * t + 1], a.tlist[4 * t + 2], &gx, &gy, &gz);
   uint32_t* mslot = a.smask + (size_t)p * a.ntlist + t;
-  if (a.fl_cur[n] != FL_LIST || (a.key_cur[n] >> 3) != (uint32_t)p) {
+  const uint8_t fn = a.fl_cur[n];
+  const uint32_t kn = a.key_cur[n];
+  if ((fn != FL_LIST) | ((kn >> 3) != (uint32_t)p)) {   /* both loads issued together */
     *mslot = 0u;
     return;
   }
@@ -523,20 +525,27 @@ __device__ __forceinline__ void k_wetting_body(const PartArgs& a, int p) {
  * of its surface nodes. */
 /* the particle's own template node t: MINDENS (every template node is
  * non-fluid) */
-__device__ __forceinline__ void shell_zero(const PartArgs& a, int p, const int c[3], int t) {
-  const size_t n = node_at(a, c[0], c[1], c[2], a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
+/* an entry (dx, dy, dz, value) of a (4, n) int8 offset table as one word */
+__device__ __forceinline__ uint32_t off_word(const int8_t* tbl, int t) {
+  return reinterpret_cast<const uint32_t*>(tbl)[t];
+}
+__device__ __forceinline__ int off_c(uint32_t w, int k) { return (int)(int8_t)((w >> (8 * k)) & 0xffu); }
+
+__device__ __forceinline__ void shell_zero(const PartArgs& a, int p, const int c[3], uint32_t w) {
+  const size_t n = node_at(a, c[0], c[1], c[2], off_c(w, 0), off_c(w, 1), off_c(w, 2));
   if ((a.key_cur[n] >> 3) != (uint32_t)p) return;
   a.rho_r[n] = 0.0f;
   a.rho_b[n] = 0.0f;
   a.phi[n] = (0.0f - 0.0f) / (0.0f + 0.0f);
 }
 
-/* ring node t around the particle's surface: densities of a fluid node from
- * this step's populations.  A node filled this step (fluid now, not on the
- * previous level) keeps the belt densities make_fluid gave it. */
+/* ring node (offset word w) around the particle's surface: densities of a
+ * fluid node from this step's populations.  A node filled this step (fluid
+ * now, not on the previous level) keeps the belt densities make_fluid gave
+ * it. */
 template <class S>
-__device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], int t) {
-  const size_t n = node_at(a, c[0], c[1], c[2], a.ring[4 * t], a.ring[4 * t + 1], a.ring[4 * t + 2]);
+__device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], uint32_t w) {
+  const size_t n = node_at(a, c[0], c[1], c[2], off_c(w, 0), off_c(w, 1), off_c(w, 2));
   const uint8_t f = a.fl_cur[n], fo = a.fl_old[n];
   float rr, rb;
   rho_sum<S>(a.pop_src, a.N, n, a.rho0, &rr, &rb);   /* issued with the type loads */
@@ -561,10 +570,21 @@ __device__ __forceinline__ void k_shellwet_body(const PartArgs& a, int p) {
   k_fill_release_body(a, p);
   int c[3];
   centre(a.pos_cur, p, c);
-  for (int t = threadIdx.x; t < a.ntmpl; t += TPB) shell_zero(a, p, c, t);
-  for (int t = threadIdx.x; t < a.nring; t += TPB) shell_ring<S>(a, c, t);
+  /* template and ring in one loop, so their loads are in flight together;
+     the offset words of the next iteration are fetched ahead */
+  const int nmax = max(a.ntmpl, a.nring);
+  int t = threadIdx.x;
+  uint32_t wt = t < a.ntmpl ? off_word(a.tmpl, t) : 0u, wr = t < a.nring ? off_word(a.ring, t) : 0u;
+  for (; t < nmax; t += TPB) {
+    const int tn = t + TPB;
+    const uint32_t wtn = tn < a.ntmpl ? off_word(a.tmpl, tn) : 0u, wrn = tn < a.nring ? off_word(a.ring, tn) : 0u;
+    if (t < a.ntmpl) shell_zero(a, p, c, wt);
+    if (t < a.nring) shell_ring<S>(a, c, wr);
+    wt = wtn;
+    wr = wrn;
+  }
   __syncthreads();
-  for (int t = threadIdx.x; t < a.ntlist; t += TPB) surface_node<S>(a, p, c, t);
+  for (int u = threadIdx.x; u < a.ntlist; u += TPB) surface_node<S>(a, p, c, u);
 }
 
 /* ---------------- bounce-back on surface links ---------------------------
