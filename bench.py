@@ -199,6 +199,7 @@ def main():
     if r16:
         out["fp16_store"] = {"mlups": round(r16["mlups"], 1), "ms_per_step": round(r16["ms_per_step"], 4),
                              "speedup_vs_f32": round(r16["mlups"] / r32["mlups"], 3),
+                             "kernel_speedup_vs_f32": round(r32["kernel_ms"] / r16["kernel_ms"], 3),
                              "kernel_achieved_gbs": round(r16["achieved_gbs"], 1),
                              "kernel_frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
                              "kernel_ms": round(r16["kernel_ms"], 4), "kernel": r16["kernel"],

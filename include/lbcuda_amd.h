@@ -108,7 +108,9 @@ typedef struct lbm_ctx lbm_ctx;
 void lbm_config_default(lbm_config* cfg, int32_t n);
 
 /* Allocate device state for one box.  Replaces setupGPUgridAndAlloc
- * (main.CUF:2133-2407) and SetConstants (main.CUF:1043-1162). */
+ * (main.CUF:2133-2407) and SetConstants (main.CUF:1043-1162).  Returns
+ * LBM_ERR_ARG for boxes beyond 2^27 nodes (512^3) with float storage or
+ * 2^28 nodes with 16-bit storage (one direction plane up to 1 GiB). */
 int lbm_create(const lbm_config* cfg, lbm_ctx** out);
 
 /* Release everything lbm_create allocated. */

@@ -214,9 +214,11 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
   }
   /* 32-bit node offsets inside the kernels */
   if ((double)cfg->nx * cfg->ny * cfg->nz > (double)(1u << 30)) return LBM_ERR_ARG;
-  /* the fused kernel addresses a node of a direction plane with a 32-bit byte
-     offset: N population pairs of 2 * esize bytes must stay below 4 GiB */
-  if ((double)cfg->nx * cfg->ny * cfg->nz * (cfg->storage == LBM_STORE_F32 ? 8.0 : 4.0) >= 4294967296.0)
+  /* the fused kernel addresses a time level through buffer resources over
+     groups of four direction planes with 32-bit offsets: four planes of N
+     population pairs (2 * esize bytes each) must fit in 4 GiB, i.e. up to
+     512^3 nodes with float storage and 645^3 with 16-bit storage */
+  if ((double)cfg->nx * cfg->ny * cfg->nz * (cfg->storage == LBM_STORE_F32 ? 8.0 : 4.0) > 1073741824.0)
     return LBM_ERR_ARG;
   lbm_ctx* c = new (std::nothrow) lbm_ctx;
   if (!c) return LBM_ERR_ALLOC;
