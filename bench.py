@@ -160,9 +160,11 @@ def main():
     log(f"inputs: {args.n}^3, {args.particles} spheres")
     rr, rb, pos = wl.pickering_inputs(args.n, args.particles)
 
-    from lbcuda_amd.lib import copy_bandwidth
+    from lbcuda_amd.lib import copy_bandwidth, stream_bandwidth
     copy_gbs = copy_bandwidth(2 << 30, 10)
     log(f"device copy kernel: {copy_gbs:.0f} GB/s (read + write)")
+    stream_gbs = stream_bandwidth(args.n ** 3, 10)
+    log(f"19-plane stream with the push's x shifts: {stream_gbs:.0f} GB/s (read + write)")
     r32 = run_gpu(args, 0, rr, rb, pos)
     r16 = None if args.no_fp16 else run_gpu(args, 1, rr, rb, pos)
     ens = None if args.no_ensemble else run_ensemble(args)
@@ -193,6 +195,7 @@ def main():
                      "kernel_ms": round(r32["kernel_ms"], 4)},
         "mlups_ref_style": round(r32["mlups"] * ((args.n + 2) / args.n) ** 3, 1),
         "copy_kernel_gbs": round(copy_gbs, 1),
+        "stream19_gbs": round(stream_gbs, 1),
         "cpu_baseline": cpu,
         "ensemble": ens,
     }

@@ -85,6 +85,18 @@ module lbcuda_amd
       type(c_ptr), value :: ctx, rho_r, rho_b, vel, phase
     end function
 
+    integer(c_int) function lbm_get_pops(ctx, color, pops) bind(C, name="lbm_get_pops")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: color
+      type(c_ptr), value :: pops
+    end function
+
+    integer(c_int) function lbm_get_rho(ctx, rho_r, rho_b) bind(C, name="lbm_get_rho")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, rho_r, rho_b
+    end function
+
     integer(c_int) function lbm_get_particles(ctx, pos, force) bind(C, name="lbm_get_particles")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, pos, force
@@ -94,6 +106,24 @@ module lbcuda_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: sums(5)
+    end function
+
+    integer(c_int) function lbm_checkpoint_size(ctx, nbytes) bind(C, name="lbm_checkpoint_size")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: nbytes
+    end function
+
+    integer(c_int) function lbm_checkpoint_save(ctx, buf, nbytes) bind(C, name="lbm_checkpoint_save")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx, buf
+      integer(c_int64_t), value :: nbytes
+    end function
+
+    integer(c_int) function lbm_checkpoint_load(ctx, buf, nbytes) bind(C, name="lbm_checkpoint_load")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx, buf
+      integer(c_int64_t), value :: nbytes
     end function
 
     integer(c_int) function lbm_status(ctx, code, step, msg, msg_len) bind(C, name="lbm_status")

@@ -8,7 +8,11 @@
 ! optional namelist file lbcuda.nml:
 !   &lbcuda box=256, taur=1.0, taub=1.0, sigma=0.1, beta=0.999, storage='f32',
 !           wetting='janus', radius=5.5, rmass=6.6667e-4, lubrication=.false.,
-!           seed=1234, autotune=.false. /
+!           seed=1234, autotune=.false., restart_every=0 /
+! Output follows OutputVTK (main.CUF:1812-1929): at step 0 and every nIterVTK
+! steps, rhoB ('rho2'), phase, rho ('rho1'), vel and the spheres go to
+! output/ in the reference's formats.  restart_every > 0 writes the initType 4
+! files (plus the sphere state) to restart_<step>/ every that many steps.
 ! The time loop replaces the chevron launches of main.CUF:125-308 with
 ! lbm_step calls of nIterOut steps and prints the reference's progress line
 ! (main.CUF:282-289; Mlup/s counts (n+2)^3 nodes there) plus the
@@ -16,10 +20,11 @@
 program lbcuda_host
   use iso_c_binding
   use lbcuda_amd
+  use lbcuda_io
   implicit none
 
   integer :: nIter, nIterOut, nIterVTK, initType, numAtoms, found, itest, i
-  integer :: box, seed
+  integer :: box, seed, restart_every
   real :: vx, vy, vz, ext_f(3), ext_t(3), max_vel
   logical :: withParticles, lrotate, lubrication, lexist
   character(len=120) :: string
@@ -30,22 +35,24 @@ program lbcuda_host
   integer(c_int) :: rc
   integer(c_int32_t) :: code, estep
   character(kind=c_char) :: msg(256)
-  real(c_float), allocatable, target :: rhoR(:,:,:), rhoB(:,:,:)
+  real(c_float), allocatable, target :: rhoR(:,:,:), rhoB(:,:,:), popR(:,:,:,:), popB(:,:,:,:)
   real(c_double), allocatable :: pos(:,:)
   real(c_double) :: sums(5)
   real(c_float) :: u0(3)
   integer(8) :: c0, c1, crate
   real(8) :: secs
-  integer :: done, chunk
+  integer :: done, chunk, nacc
+  real(8) :: tacc
   logical :: autotune
   integer(c_int32_t) :: tile, zchunk
   real(c_double) :: best_ms
-  namelist /lbcuda/ box, taur, taub, sigma, beta, storage, wetting, radius, rmass, lubrication, seed, autotune
+  namelist /lbcuda/ box, taur, taub, sigma, beta, storage, wetting, radius, rmass, lubrication, seed, autotune, &
+                     restart_every
 
   ! defaults: dimensions_m.CUF:7 (MYDIMESION 64), :50, :59-60, :103-104
   box = 64; taur = 1.0; taub = 1.0; sigma = 0.1; beta = 0.999
   storage = 'f32'; wetting = 'janus'; radius = 5.5; rmass = 1.0/1500.0
-  lubrication = .false.; seed = 1234; autotune = .false.
+  lubrication = .false.; seed = 1234; autotune = .false.; restart_every = 0
 
   inquire(file='lbcuda.nml', exist=lexist)
   if (lexist) then
@@ -150,7 +157,14 @@ program lbcuda_host
 
   ! ---- particles: read_input_atom, main.CUF:1230-1350 ----
   if (withParticles) then
-    call read_input_atom()
+    allocate(pos(13, numAtoms))
+    lexist = .false.
+    if (initType == 4) lexist = read_particles_restart('particles.init.bin', pos, numAtoms)
+    if (lexist) then
+      write(*,*) 'particles.init.bin] Using the saved state of', numAtoms, ' particles'
+    else
+      call read_input_atom()
+    end if
     rc = lbm_set_particles(ctx, int(numAtoms, c_int32_t), pos)
     call check(rc, 'lbm_set_particles')
   end if
@@ -163,6 +177,12 @@ program lbcuda_host
   case (3)
     call read_rho_files()
     rc = lbm_init_fields(ctx, LBM_INIT_RHO, u0, c_loc(rhoR), c_loc(rhoB), c_null_ptr)
+  case (4)
+    call read_rho_files()
+    rc = lbm_init_fields(ctx, LBM_INIT_POPS, u0, c_loc(rhoR), c_loc(rhoB), c_null_ptr)
+    call check(rc, 'lbm_init_fields')
+    call read_pops_files()
+    rc = lbm_set_pops(ctx, c_loc(popR), c_loc(popB))
   case (6)
     allocate(rhoR(cfg%nx, cfg%ny, cfg%nz), rhoB(cfg%nx, cfg%ny, cfg%nz))
     rc = lbm_fill_random_rho(int(seed, c_int64_t), 0.5d0, 1.0d-4, cfg%nx, cfg%ny, cfg%nz, &
@@ -174,6 +194,10 @@ program lbcuda_host
     stop 1
   end select
   call check(rc, 'lbm_init_fields')
+  if (initType == 4) then
+    inquire(file='state.ckpt', exist=lexist)
+    if (lexist) call load_checkpoint('state.ckpt')
+  end if
 
   ! tile of the collide-stream kernel: the reference fixes TILE1..3 at compile
   ! time (defines.h:14-24); here the library times its tile table on this box
@@ -187,11 +211,18 @@ program lbcuda_host
   write(*,'(a,2es16.8)') ' initial mass R,B:', sums(1), sums(2)
   write(*,*) 'Run...'
 
-  ! ---- step loop, main.CUF:125-308 ----
   call system_clock(count_rate=crate)
-  done = 0
+  call execute_command_line('mkdir -p output')
+  if (nIterVTK > 0) call output_vtk(0)      ! main.CUF:114
+
+  ! ---- step loop, main.CUF:125-308 ----
+  ! lbm_step runs up to the next progress line, VTK output or restart dump;
+  ! the progress line times the steps only (output is not in the interval)
+  done = 0; nacc = 0; tacc = 0.0d0
   do while (done < nIter)
-    chunk = min(nIterOut, nIter - done)
+    chunk = min(nIterOut - mod(done, nIterOut), nIter - done)
+    if (nIterVTK > 0) chunk = min(chunk, nIterVTK - mod(done, nIterVTK))
+    if (restart_every > 0) chunk = min(chunk, restart_every - mod(done, restart_every))
     call system_clock(c0)
     rc = lbm_step(ctx, int(chunk, c_int32_t))
     call system_clock(c1)
@@ -201,11 +232,22 @@ program lbcuda_host
       write(*,'(a,i4,a,i10,a,256a)') ' lbm_step error ', code, ' at step', estep, ': ', msg(1:cstrlen(msg))
       stop 10
     end if
-    secs = real(c1 - c0, 8) / real(crate, 8)
-    write(*,100) done, secs, secs*1.0d3/chunk, &
-        real(cfg%nx + 2, 8) * real(cfg%ny + 2, 8) * real(cfg%nz + 2, 8) * chunk / secs / 1.0d6, &
-        real(cfg%nx, 8) * real(cfg%ny, 8) * real(cfg%nz, 8) * chunk / secs / 1.0d6
-100 format('Iter:',I10,'   Time(s):', G10.4, '   ms/it:',G9.3, '   Mlup/s:',F8.0, '   interior Mlup/s:', F8.0)
+    nacc = nacc + chunk
+    tacc = tacc + real(c1 - c0, 8) / real(crate, 8)
+    if (mod(done, nIterOut) == 0 .or. done == nIter) then
+      secs = tacc
+      write(*,100) done, secs, secs*1.0d3/nacc, &
+          real(cfg%nx + 2, 8) * real(cfg%ny + 2, 8) * real(cfg%nz + 2, 8) * nacc / secs / 1.0d6, &
+          real(cfg%nx, 8) * real(cfg%ny, 8) * real(cfg%nz, 8) * nacc / secs / 1.0d6
+100   format('Iter:',I10,'   Time(s):', G10.4, '   ms/it:',G9.3, '   Mlup/s:',F8.0, '   interior Mlup/s:', F8.0)
+      nacc = 0; tacc = 0.0d0
+    end if
+    if (nIterVTK > 0) then
+      if (mod(done, nIterVTK) == 0) call output_vtk(done)   ! main.CUF:303-306
+    end if
+    if (restart_every > 0) then
+      if (mod(done, restart_every) == 0) call write_restart(done)
+    end if
   end do
 
   rc = lbm_get_sums(ctx, sums)
@@ -269,7 +311,6 @@ contains
       stop 1
     end if
     read(13, '(a120)') string
-    allocate(pos(13, numAtoms))
     pos = 0.0d0
     do l = 1, numAtoms
       if (.not. lrotate) then
@@ -287,24 +328,110 @@ contains
 
   ! read_and_setRed_Blue, main.CUF:2444-2479: unformatted, 6-int extent header
   subroutine read_rho_files()
-    integer :: minx, maxx, miny, maxy, minz, maxz
     allocate(rhoR(cfg%nx, cfg%ny, cfg%nz), rhoB(cfg%nx, cfg%ny, cfg%nz))
-    open(unit=14, file='rhoR.init.bin', form='unformatted', action='read', status='old')
-    read(14) minx, maxx, miny, maxy, minz, maxz
-    if (minx /= 1 .or. miny /= 1 .or. minz /= 1 .or. maxx /= cfg%nx .or. maxy /= cfg%ny .or. maxz /= cfg%nz) then
-      write(*,'(a,6i8)') 'Wrong start rhoR file', minx, maxx, miny, maxy, minz, maxz
-      stop 1
+    call read_init_bin('rhoR.init.bin', 'rhoR', rhoR, cfg%nx, cfg%ny, cfg%nz)
+    call read_init_bin('rhoB.init.bin', 'rhoB', rhoB, cfg%nx, cfg%ny, cfg%nz)
+  end subroutine
+
+  ! read_and_setPops, main.CUF:2481-2529: popR.<l>.init.bin, popB.<l>.init.bin
+  subroutine read_pops_files()
+    integer :: l
+    allocate(popR(cfg%nx, cfg%ny, cfg%nz, 0:18), popB(cfg%nx, cfg%ny, cfg%nz, 0:18))
+    do l = 0, 18
+      call read_init_bin('popR.' // fmtnumb(l) // '.init.bin', 'popsR', popR(:,:,:,l), cfg%nx, cfg%ny, cfg%nz)
+      call read_init_bin('popB.' // fmtnumb(l) // '.init.bin', 'popsB', popB(:,:,:,l), cfg%nx, cfg%ny, cfg%nz)
+    end do
+  end subroutine
+
+  ! exact state image of the library (lbm_checkpoint_save), when it has one
+  subroutine save_checkpoint(path)
+    character(len=*), intent(in) :: path
+    integer(c_int64_t) :: nbytes
+    integer(c_int8_t), allocatable, target :: img(:)
+    integer :: u
+    if (lbm_checkpoint_size(ctx, nbytes) /= 0) return
+    allocate(img(nbytes))
+    rc = lbm_checkpoint_save(ctx, c_loc(img), nbytes)
+    call check(rc, 'lbm_checkpoint_save')
+    open(newunit=u, file=path, status='replace', access='stream', form='unformatted', action='write')
+    write(u) img
+    close(u)
+  end subroutine
+
+  ! initType 4 with a state.ckpt beside the restart files: continue exactly
+  subroutine load_checkpoint(path)
+    character(len=*), intent(in) :: path
+    integer(c_int64_t) :: nbytes
+    integer(c_int8_t), allocatable, target :: img(:)
+    integer :: u
+    inquire(file=path, size=nbytes)
+    allocate(img(nbytes))
+    open(newunit=u, file=path, status='old', access='stream', form='unformatted', action='read')
+    read(u) img
+    close(u)
+    rc = lbm_checkpoint_load(ctx, c_loc(img), nbytes)
+    call check(rc, 'lbm_checkpoint_load')
+    write(*,*) 'state.ckpt] Continuing from the saved state'
+  end subroutine
+
+  ! OutputVTK, main.CUF:1812-1929: colour densities as the device keeps them
+  ! (rhoR_d, rhoB_d), phase and velocity from the populations (moments2Fl)
+  subroutine output_vtk(step)
+    integer, intent(in) :: step
+    real(c_float), allocatable, target :: rr(:,:,:), rb(:,:,:), ph(:,:,:), vel(:,:,:,:)
+    real(c_double), allocatable, target :: p(:,:)
+    integer(8) :: t0, t1
+    call system_clock(t0)
+    allocate(rr(cfg%nx, cfg%ny, cfg%nz), rb(cfg%nx, cfg%ny, cfg%nz), ph(cfg%nx, cfg%ny, cfg%nz), &
+             vel(3, cfg%nx, cfg%ny, cfg%nz))
+    rc = lbm_get_moments(ctx, c_null_ptr, c_null_ptr, c_loc(vel), c_loc(ph))
+    call check(rc, 'lbm_get_moments')
+    rc = lbm_get_rho(ctx, c_loc(rr), c_loc(rb))
+    call check(rc, 'lbm_get_rho')
+    call write_vti('rhoB', step, 'rho2', 1, rb, cfg%nx, cfg%ny, cfg%nz)
+    call write_vti('phase', step, 'phase', 1, ph, cfg%nx, cfg%ny, cfg%nz)
+    call write_vti('rho', step, 'rho1', 1, rr, cfg%nx, cfg%ny, cfg%nz)
+    call write_vti('vel', step, 'vel', 3, vel, cfg%nx, cfg%ny, cfg%nz)
+    if (withParticles) then
+      allocate(p(13, numAtoms))
+      rc = lbm_get_particles(ctx, c_loc(p), c_null_ptr)
+      call check(rc, 'lbm_get_particles')
+      call write_particle_vtk('particle', step, p, numAtoms, lrotate)
     end if
-    read(14) rhoR
-    close(14)
-    open(unit=14, file='rhoB.init.bin', form='unformatted', action='read', status='old')
-    read(14) minx, maxx, miny, maxy, minz, maxz
-    if (minx /= 1 .or. miny /= 1 .or. minz /= 1 .or. maxx /= cfg%nx .or. maxy /= cfg%ny .or. maxz /= cfg%nz) then
-      write(*,'(a,6i8)') 'Wrong start rhoB file', minx, maxx, miny, maxy, minz, maxz
-      stop 1
+    call system_clock(t1)
+    write(*,'(a,g10.2,a)') 'Time for VTK:', real(t1 - t0, 8) / real(crate, 8), ' secs'
+  end subroutine
+
+  ! initType 4 input files of the current state, in restart_<step>/
+  subroutine write_restart(step)
+    integer, intent(in) :: step
+    character(len=:), allocatable :: dir
+    real(c_float), allocatable, target :: a(:,:,:), b(:,:,:), q(:,:,:,:)
+    real(c_double), allocatable, target :: p(:,:)
+    integer :: l, color
+    dir = 'restart_' // fmtnumb(step)
+    call execute_command_line('mkdir -p ' // dir)
+    allocate(a(cfg%nx, cfg%ny, cfg%nz), b(cfg%nx, cfg%ny, cfg%nz), q(cfg%nx, cfg%ny, cfg%nz, 0:18))
+    rc = lbm_get_rho(ctx, c_loc(a), c_loc(b))
+    call check(rc, 'lbm_get_rho')
+    call write_init_bin(dir // '/rhoR.init.bin', a, cfg%nx, cfg%ny, cfg%nz)
+    call write_init_bin(dir // '/rhoB.init.bin', b, cfg%nx, cfg%ny, cfg%nz)
+    do color = 0, 1
+      rc = lbm_get_pops(ctx, int(color, c_int32_t), c_loc(q))
+      call check(rc, 'lbm_get_pops')
+      do l = 0, 18
+        call write_init_bin(dir // '/pop' // merge('R', 'B', color == 0) // '.' // fmtnumb(l) // '.init.bin', &
+                            q(:,:,:,l), cfg%nx, cfg%ny, cfg%nz)
+      end do
+    end do
+    if (withParticles) then
+      allocate(p(13, numAtoms))
+      rc = lbm_get_particles(ctx, c_loc(p), c_null_ptr)
+      call check(rc, 'lbm_get_particles')
+      call write_particles_restart(dir, p, numAtoms)
     end if
-    read(14) rhoB
-    close(14)
+    call save_checkpoint(dir // '/state.ckpt')
+    write(*,'(a,i10,a)') ' restart files of step', step, ' written to ' // dir
   end subroutine
 
 end program lbcuda_host

@@ -163,6 +163,20 @@ int lbm_get_particles(lbm_ctx* ctx, double* pos, double* force);
  * out[0]=sum rhoR, out[1]=sum rhoB, out[2..4]=fluid momentum. */
 int lbm_get_sums(lbm_ctx* ctx, double out[5]);
 
+/* Checkpoint of the whole box state as one opaque byte image in the storage
+ * format of the box (no reference counterpart: the reference reads initType
+ * 3/4 restart files, main.CUF:2444-2529, but writes none, and keeps no
+ * particle restart).  size gives the image's bytes; save fills a host buffer
+ * of at least that size; load restores it into a context created with an
+ * identical lbm_config (and, with particles, after lbm_set_particles with the
+ * same count), after which lbm_step continues bit for bit as the saved box
+ * would have.  The closure block (lbm_set_closure) is configuration and is
+ * not part of the image.  LBM_ERR_ARG for a foreign or truncated image or a
+ * different config, LBM_ERR_STATE for a different particle count. */
+int lbm_checkpoint_size(lbm_ctx* ctx, int64_t* bytes);
+int lbm_checkpoint_save(lbm_ctx* ctx, void* buf, int64_t bytes);
+int lbm_checkpoint_load(lbm_ctx* ctx, const void* buf, int64_t bytes);
+
 /* Error state: code (LBM_ERR_*), step at which it was raised, message. */
 int lbm_status(lbm_ctx* ctx, int32_t* code, int32_t* step, char* msg,
                int32_t msg_len);
@@ -262,6 +276,13 @@ int lbm_ensemble_autotune(lbm_ensemble* ens, int32_t reps, int32_t* tile, int32_
  * The reference probes host transfer bandwidth at start-up
  * (main.CUF:2396-2404); this is the HBM ceiling the step is compared with. */
 int lbm_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs);
+
+/* Bandwidth in GB/s (read + write bytes over time) of the fused step's access
+ * pattern without its arithmetic, over `nodes` nodes (<= 2^27): 19 direction
+ * planes of float pairs read and 19 written, the planes with e_x = +-1
+ * shifted by one node (the push).  No reference counterpart; the ceiling the
+ * fused kernel's measured traffic is compared with. */
+int lbm_stream_bandwidth(int64_t nodes, int32_t reps, double* gbs);
 
 /* Library build identification (gfx target, storage formats). */
 const char* lbm_version(void);

@@ -548,6 +548,130 @@ int lbm_get_sums(lbm_ctx* c, double out[5]) {
   return rc;
 }
 
+/* ---- checkpoint image --------------------------------------------------
+ * Header, then every device buffer that carries state from one step to the
+ * next, in a fixed order: the populations of both levels, the flags,
+ * densities and phase field, and, with particles, both time levels of the
+ * particle state, the force accumulators, the stamped centres, the owner keys
+ * and the fill claims.  Everything else a step touches is rebuilt inside it. */
+namespace {
+
+struct CkptHeader {
+  char magic[8];          /* "LBCKPT1\0"                                   */
+  lbm_config cfg;         /* must equal the loading context's config       */
+  int32_t np, cur;
+  int64_t step;
+  int64_t bytes;          /* whole image                                   */
+};
+
+struct CkptPart {
+  void* dev;
+  size_t bytes;
+};
+
+void ckpt_parts(lbm_ctx* c, std::vector<CkptPart>& v) {
+  v.clear();
+  const size_t N = c->N;
+  /* both levels: slots that no node pushes into (those next to a sphere's
+     interior) keep the value of two steps before, and the surface densities
+     read them */
+  for (int b = 0; b < 2; ++b) v.push_back({c->pop[b], 2 * 19 * N * c->esize});
+  for (int b = 0; b < 2; ++b) v.push_back({c->flags[b], N});
+  v.push_back({c->rho_r, sizeof(float) * N});
+  v.push_back({c->rho_b, sizeof(float) * N});
+  v.push_back({c->phi, sizeof(float) * N});
+  if (c->np > 0) {
+    const size_t n = (size_t)c->np;
+    for (int b = 0; b < 2; ++b) {
+      v.push_back({c->d_pos[b], sizeof(double) * 13 * n});
+      v.push_back({c->d_force[b], sizeof(double) * 12 * n});
+      v.push_back({c->d_stamp[b], sizeof(int) * 3 * n});
+      v.push_back({c->d_key[b], sizeof(uint32_t) * N});
+    }
+    v.push_back({c->d_myf, sizeof(double) * 3 * n});
+    v.push_back({c->d_myf2, sizeof(double) * 3 * n});
+    v.push_back({c->d_myt, sizeof(double) * 3 * n});
+    v.push_back({c->d_myt2, sizeof(double) * 3 * n});
+    v.push_back({c->d_claim, sizeof(int) * N});
+  }
+}
+
+int64_t ckpt_bytes(lbm_ctx* c) {
+  int64_t s = sizeof(CkptHeader);
+  std::vector<CkptPart> v;
+  ckpt_parts(c, v);
+  for (const CkptPart& p : v) s += (int64_t)p.bytes;
+  return s;
+}
+
+}  // namespace
+
+int lbm_checkpoint_size(lbm_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return LBM_ERR_ARG;
+  if (!c->initialised) return LBM_ERR_STATE;
+  *bytes = ckpt_bytes(c);
+  return 0;
+}
+
+int lbm_checkpoint_save(lbm_ctx* c, void* buf, int64_t bytes) {
+  if (!c || !buf) return LBM_ERR_ARG;
+  if (!c->initialised) return LBM_ERR_STATE;
+  if (bytes < ckpt_bytes(c)) return LBM_ERR_ARG;
+  CkptHeader h;
+  std::memset(&h, 0, sizeof h);
+  std::memcpy(h.magic, "LBCKPT1", 8);
+  h.cfg = c->cfg;
+  h.np = c->np;
+  h.cur = c->cur;
+  h.step = c->step;
+  h.bytes = ckpt_bytes(c);
+  std::memcpy(buf, &h, sizeof h);
+  char* p = static_cast<char*>(buf) + sizeof h;
+  int rc = hip_ok(hipStreamSynchronize(c->stream));
+  std::vector<CkptPart> parts;
+  ckpt_parts(c, parts);
+  for (const CkptPart& s : parts) {
+    if (rc) break;
+    rc |= hip_ok(hipMemcpy(p, s.dev, s.bytes, hipMemcpyDeviceToHost));
+    p += s.bytes;
+  }
+  return rc;
+}
+
+int lbm_checkpoint_load(lbm_ctx* c, const void* buf, int64_t bytes) {
+  if (!c || !buf || bytes < (int64_t)sizeof(CkptHeader)) return LBM_ERR_ARG;
+  CkptHeader h;
+  std::memcpy(&h, buf, sizeof h);
+  if (std::memcmp(h.magic, "LBCKPT1", 8) != 0) return LBM_ERR_ARG;
+  if (std::memcmp(&h.cfg, &c->cfg, sizeof(lbm_config)) != 0) return LBM_ERR_ARG;
+  if (h.cur != 0 && h.cur != 1) return LBM_ERR_ARG;
+  /* particle buffers exist once lbm_set_particles has run with the same count */
+  if (h.np != c->np) return LBM_ERR_STATE;
+  const int cur = c->cur;
+  c->cur = h.cur;
+  const int64_t need = ckpt_bytes(c);
+  c->cur = cur;
+  if (h.bytes != need || bytes < need) return LBM_ERR_ARG;
+  int rc = hip_ok(hipStreamSynchronize(c->stream));
+  if (rc) return rc;
+  c->cur = h.cur;
+  const char* p = static_cast<const char*>(buf) + sizeof h;
+  std::vector<CkptPart> parts;
+  ckpt_parts(c, parts);
+  for (const CkptPart& s : parts) {
+    if (rc) break;
+    rc |= hip_ok(hipMemcpy(s.dev, p, s.bytes, hipMemcpyHostToDevice));
+    p += s.bytes;
+  }
+  rc |= hip_ok(hipMemsetAsync(c->d_status, 0, sizeof(lbm::DevStatus), c->stream));
+  rc |= hip_ok(hipStreamSynchronize(c->stream));
+  if (rc) return rc;
+  c->step = h.step;
+  c->host_err = 0;
+  c->initialised = true;
+  return 0;
+}
+
 int lbm_status(lbm_ctx* c, int32_t* code, int32_t* step, char* msg, int32_t len) {
   if (!c) return LBM_ERR_ARG;
   lbm::DevStatus st{0, 0};
@@ -772,6 +896,35 @@ int lbm_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
   (void)hipFree(b);
   if (rc) return LBM_ERR_HIP;
   *gbs = 2.0 * (double)(n4 * 16) * reps / ((double)ms * 1e-3) / 1e9;
+  return LBM_OK;
+}
+
+int lbm_stream_bandwidth(int64_t nodes, int32_t reps, double* gbs) {
+  if (nodes < 256 || nodes > ((int64_t)1 << 27) || reps <= 0 || !gbs) return LBM_ERR_ARG;
+  const size_t bytes = (size_t)19 * (size_t)nodes * sizeof(float) * 2;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return LBM_ERR_ALLOC;
+  if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return LBM_ERR_ALLOC; }
+  hipStream_t st;
+  int rc = hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  rc |= hip_ok(hipEventCreate(&e0));
+  rc |= hip_ok(hipEventCreate(&e1));
+  rc |= hip_ok(hipMemsetAsync(a, 0, bytes, st));
+  rc |= lbm_launch_stream_probe(a, b, (uint32_t)nodes, st);   /* warm-up */
+  rc |= hip_ok(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) rc |= lbm_launch_stream_probe(a, b, (uint32_t)nodes, st);
+  rc |= hip_ok(hipEventRecord(e1, st));
+  rc |= hip_ok(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  rc |= hip_ok(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return LBM_ERR_HIP;
+  *gbs = 2.0 * (double)bytes * reps / ((double)ms * 1e-3) / 1e9;
   return LBM_OK;
 }
 

@@ -325,6 +325,7 @@ bool lbm_fused_tile_valid(const lbm_ctx* c, int tile, int zc);
 void lbm_fused_tile_shape(int tile, int* tx, int* ty, int* wx);
 int lbm_launch_rho_from_pops(lbm_ctx* c, float* d_rr, float* d_rb);
 int lbm_launch_copy_probe(const void* a, void* b, size_t n_float4, hipStream_t st);
+int lbm_launch_stream_probe(const void* a, void* b, uint32_t nodes, hipStream_t st);
 int lbm_launch_init_eq(lbm_ctx* c, int type, const float u0[3], const float* d_vel);
 int lbm_launch_moments(lbm_ctx* c, float* d_rr, float* d_rb, float* d_vel, float* d_phase);
 int lbm_launch_sums(lbm_ctx* c, double* d_out);

@@ -369,6 +369,31 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a
     b[i] = a[i];
 }
 
+/* The access pattern of the fused step without its arithmetic: each lane
+ * reads the pairs of 19 direction planes of one node and writes 19 pairs,
+ * the ones with e_x = +-1 shifted by one node along x (as the push does). */
+__global__ __launch_bounds__(256) void k_stream_probe(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                      uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  float2 v[19];
+#pragma unroll
+  for (int l = 0; l < 19; ++l) v[l] = src[(size_t)l * n + i];
+  float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+  for (int l = 0; l < 19; ++l) {
+    sx += v[l].x;
+    sy += v[l].y;
+  }
+#pragma unroll
+  for (int l = 0; l < 19; ++l) {
+    uint32_t j = i;
+    if (EX(l) > 0 && i + 1 < n) j = i + 1;
+    if (EX(l) < 0 && i > 0) j = i - 1;
+    dst[(size_t)l * n + j] = make_float2(v[l].x * 0.5f + sx * 1e-9f, v[l].y * 0.5f + sy * 1e-9f);
+  }
+}
+
 FluidArgs make_args(lbm_ctx* c) {
   FluidArgs a;
   a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
@@ -470,6 +495,12 @@ int lbm_launch_pack_pops(lbm_ctx* c, const float* pr, const float* pb) {
   a.dst = c->pop[c->cur];
   LBM_STORAGE_SWITCH(c->storage,
     hipLaunchKernelGGL(k_pack<S>, dim3(nblocks(c->N * 19, 256)), dim3(256), 0, c->stream, a, pr, pb))
+  return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
+}
+
+int lbm_launch_stream_probe(const void* a, void* b, uint32_t nodes, hipStream_t st) {
+  hipLaunchKernelGGL(k_stream_probe, dim3((nodes + 255u) / 256u), dim3(256), 0, st, (const float2*)a, (float2*)b,
+                     nodes);
   return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
 }
 

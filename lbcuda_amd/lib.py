@@ -53,6 +53,9 @@ def lib():
     L.lbm_get_flags.argtypes = [vp, vp]
     L.lbm_get_particles.argtypes = [vp, vp, vp]
     L.lbm_get_sums.argtypes = [vp, vp]
+    L.lbm_checkpoint_size.argtypes = [vp, P(ctypes.c_int64)]
+    L.lbm_checkpoint_save.argtypes = [vp, vp, ctypes.c_int64]
+    L.lbm_checkpoint_load.argtypes = [vp, vp, ctypes.c_int64]
     L.lbm_status.argtypes = [vp, P(i32), P(i32), ctypes.c_char_p, i32]
     L.lbm_kernel_timing.argtypes = [vp, i32]
     L.lbm_get_kernel_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_int64)]
@@ -64,6 +67,7 @@ def lib():
     L.lbm_tile_shape.argtypes = [i32, P(i32), P(i32), P(i32)]
     L.lbm_autotune.argtypes = [vp, i32, P(i32), P(i32), P(ctypes.c_double)]
     L.lbm_copy_bandwidth.argtypes = [ctypes.c_int64, i32, P(ctypes.c_double)]
+    L.lbm_stream_bandwidth.argtypes = [ctypes.c_int64, i32, P(ctypes.c_double)]
     L.lbm_mlp_forward.argtypes = [vp, vp, vp, i32]
     L.lbm_mlp_loss_grad.argtypes = [vp, vp, vp, i32, P(ctypes.c_double), vp]
     L.lbm_mlp_train.argtypes = [vp, vp, vp, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, P(ctypes.c_double)]
@@ -80,9 +84,10 @@ def lib():
     for name in ("lbm_create", "lbm_destroy", "lbm_init_fields", "lbm_set_pops", "lbm_set_particles",
                  "lbm_step", "lbm_get_pops", "lbm_get_rho", "lbm_get_moments", "lbm_get_flags",
                  "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
+                 "lbm_checkpoint_size", "lbm_checkpoint_save", "lbm_checkpoint_load",
                  "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_set_kernel", "lbm_get_kernel",
-                 "lbm_tile_shape", "lbm_autotune", "lbm_copy_bandwidth", "lbm_ensemble_create",
-                 "lbm_ensemble_destroy", "lbm_ensemble_box", "lbm_ensemble_step", "lbm_ensemble_set_tile",
+                 "lbm_tile_shape", "lbm_autotune", "lbm_copy_bandwidth", "lbm_stream_bandwidth",
+                 "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_box", "lbm_ensemble_step", "lbm_ensemble_set_tile",
                  "lbm_ensemble_autotune", "lbm_mlp_forward", "lbm_mlp_loss_grad", "lbm_mlp_train",
                  "lbm_set_closure"):
         getattr(L, name).restype = ctypes.c_int
@@ -95,9 +100,11 @@ EXPORTED = ["lbm_config_default", "lbm_create", "lbm_destroy", "lbm_init_fields"
             "lbm_get_flags", "lbm_get_particles", "lbm_get_sums", "lbm_status", "lbm_kernel_timing",
             "lbm_get_kernel_timing", "lbm_kernel_info", "lbm_version", "lbm_fill_random_rho",
             "lbm_set_kernel", "lbm_get_kernel", "lbm_num_tiles", "lbm_tile_shape", "lbm_autotune",
-            "lbm_copy_bandwidth", "lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_box",
+            "lbm_copy_bandwidth", "lbm_stream_bandwidth", "lbm_ensemble_create", "lbm_ensemble_destroy",
+            "lbm_ensemble_box",
             "lbm_ensemble_step", "lbm_ensemble_set_tile", "lbm_ensemble_autotune", "lbm_mlp_forward",
-            "lbm_mlp_loss_grad", "lbm_mlp_train", "lbm_set_closure"]
+            "lbm_mlp_loss_grad", "lbm_mlp_train", "lbm_set_closure", "lbm_checkpoint_size",
+            "lbm_checkpoint_save", "lbm_checkpoint_load"]
 
 
 def _f32(a):
@@ -199,6 +206,19 @@ class Box:
         out = np.zeros(5, dtype=np.float64)
         self._check(lib().lbm_get_sums(self.h, _ptr(out)), "lbm_get_sums")
         return out
+
+    def checkpoint(self):
+        """Whole box state as bytes (lbm_checkpoint_save)."""
+        n = ctypes.c_int64()
+        self._check(lib().lbm_checkpoint_size(self.h, ctypes.byref(n)), "lbm_checkpoint_size")
+        buf = np.empty(n.value, dtype=np.uint8)
+        self._check(lib().lbm_checkpoint_save(self.h, _ptr(buf), n.value), "lbm_checkpoint_save")
+        return buf
+
+    def restore(self, image):
+        """Load an image of checkpoint() into this box (lbm_checkpoint_load)."""
+        buf = np.ascontiguousarray(image, dtype=np.uint8)
+        self._check(lib().lbm_checkpoint_load(self.h, _ptr(buf), buf.size), "lbm_checkpoint_load")
 
     def status(self):
         code = ctypes.c_int32()
@@ -334,6 +354,16 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
     rc = lib().lbm_copy_bandwidth(nbytes, reps, ctypes.byref(g))
     if rc:
         raise LbmError(rc, "lbm_copy_bandwidth")
+    return g.value
+
+
+def stream_bandwidth(nodes=1 << 24, reps=10):
+    """Measured bandwidth (GB/s, read + write) of the fused step's access
+    pattern without its arithmetic (19 planes read, 19 written, x-shifted)."""
+    g = ctypes.c_double()
+    rc = lib().lbm_stream_bandwidth(nodes, reps, ctypes.byref(g))
+    if rc:
+        raise LbmError(rc, "lbm_stream_bandwidth")
     return g.value
 
 

@@ -68,22 +68,68 @@ int lbm_set_particles(lbm_ctx* c, int32_t n, const double* pos) {
 
 int lbm_step(lbm_ctx* c, int32_t n) { return orc_step(c->o, n); }
 
+/* moments2Fl (write_output.f90:975-1041): colour densities, velocity and
+ * phase field on fluid nodes with positive total density, zeros elsewhere. */
 int lbm_get_moments(lbm_ctx* c, float* rr, float* rb, float* vel, float* ph) {
   std::vector<double> pr(c->N * 19), pb(c->N * 19);
   orc_get_pops(c->o, 0, pr.data());
   orc_get_pops(c->o, 1, pb.data());
+  std::vector<int8_t> fl(c->N);
+  orc_get_flags(c->o, fl.data());
+  static const int ex[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0};
+  static const int ey[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1};
+  static const int ez[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
   for (size_t n = 0; n < c->N; ++n) {
-    double a = 0, b = 0;
-    for (int l = 0; l < 19; ++l) { a += pr[n + c->N * l]; b += pb[n + c->N * l]; }
-    if (rr) rr[n] = (float)a;
-    if (rb) rb[n] = (float)b;
-    if (ph) ph[n] = (float)((a - b) / (a + b));
-    if (vel) vel[3 * n] = vel[3 * n + 1] = vel[3 * n + 2] = 0.0f;
+    double a = 0, b = 0, j[3] = {0, 0, 0};
+    for (int l = 0; l < 19; ++l) {
+      const double r = pr[n + c->N * l], q = pb[n + c->N * l];
+      a += r;
+      b += q;
+      j[0] += (r + q) * ex[l];
+      j[1] += (r + q) * ey[l];
+      j[2] += (r + q) * ez[l];
+    }
+    const bool ok = fl[n] == 1 && a + b > 0.0;  // 1 = fluid node (lbm_get_flags)
+    if (rr) rr[n] = ok ? (float)a : 0.0f;
+    if (rb) rb[n] = ok ? (float)b : 0.0f;
+    if (ph) ph[n] = ok ? (float)((a - b) / (a + b)) : 0.0f;
+    if (vel)
+      for (int d = 0; d < 3; ++d) vel[3 * n + d] = ok ? (float)(j[d] / (a + b)) : 0.0f;
   }
   return 0;
 }
 
+int lbm_get_pops(lbm_ctx* c, int32_t color, float* out) {
+  if (!c || !out || color < 0 || color > 1) return LBM_ERR_ARG;
+  std::vector<double> p(c->N * 19);
+  int rc = orc_get_pops(c->o, color, p.data());
+  for (size_t i = 0; i < p.size(); ++i) out[i] = (float)p[i];
+  return rc;
+}
+
+int lbm_get_rho(lbm_ctx* c, float* rr, float* rb) {
+  if (!c) return LBM_ERR_ARG;
+  std::vector<double> a(c->N), b(c->N);
+  int rc = orc_get_rho(c->o, a.data(), b.data());
+  for (size_t n = 0; n < c->N; ++n) {
+    if (rr) rr[n] = (float)a[n];
+    if (rb) rb[n] = (float)b[n];
+  }
+  return rc;
+}
+
+int lbm_get_flags(lbm_ctx* c, int8_t* out) {
+  if (!c || !out) return LBM_ERR_ARG;
+  return orc_get_flags(c->o, out);
+}
+
 int lbm_get_particles(lbm_ctx* c, double* pos, double* force) { return orc_get_particles(c->o, pos, force); }
+
+/* The CPU test implementation keeps no checkpoint image: size reports
+ * LBM_ERR_ARG and callers fall back to the restart files. */
+int lbm_checkpoint_size(lbm_ctx*, int64_t*) { return LBM_ERR_ARG; }
+int lbm_checkpoint_save(lbm_ctx*, void*, int64_t) { return LBM_ERR_ARG; }
+int lbm_checkpoint_load(lbm_ctx*, const void*, int64_t) { return LBM_ERR_ARG; }
 
 int lbm_get_sums(lbm_ctx* c, double out[5]) {
   std::vector<double> pr(c->N * 19), pb(c->N * 19);
