@@ -32,7 +32,9 @@ module lbcuda_amd
     integer(c_int32_t) :: lubrication
     real(c_float)      :: lub_k, lub_rmin, lub_rcap
     integer(c_int32_t) :: closure
-    integer(c_int32_t) :: reserved(8)
+    integer(c_int32_t) :: diffdens
+    real(c_float)      :: dens_r, dens_b
+    integer(c_int32_t) :: reserved(5)
   end type lbm_config
 
   interface

@@ -98,7 +98,13 @@ typedef struct lbm_config {
   int32_t lubrication;       /* llubrication (applied to the FP64 sums)            */
   float   lub_k, lub_rmin, lub_rcap;   /* cutoff gap, minimum gap (lattice units) */
   int32_t closure;           /* learned near-contact closure (no reference part)   */
-  int32_t reserved[8];
+  /* density-ratio colour gradient (the reference's DIFFDENS build,
+   * kernels_fluid_CG.CUF:713-800, main.CUF:1047-1085): colour densities
+   * densR/densB (one of them must be 1), colour-dependent rest weights
+   * alphaCG and the psi/xi correction terms.  Runs on the two-kernel step. */
+  int32_t diffdens;
+  float   dens_r, dens_b;
+  int32_t reserved[5];
 } lbm_config;
 
 typedef struct lbm_ctx lbm_ctx;
@@ -191,7 +197,9 @@ int lbm_get_kernel_timing(lbm_ctx* ctx, double* total_ms, int64_t* launches);
  * (TILE1..TILE3, defines.h:14-24, checked in main.CUF:2137-2148); here the
  * variant (LBM_KERNEL_*), the fused kernel's tile (index into the built-in
  * tile table, see lbm_tile_shape) and its z-chunk length are run-time
- * choices.  lbm_set_kernel rejects a tile that does not fit the box. */
+ * choices.  lbm_set_kernel rejects a tile that does not fit the box, and the
+ * fused variant for a density-ratio (diffdens) box, which runs the two-kernel
+ * step only. */
 int lbm_set_kernel(lbm_ctx* ctx, int32_t variant, int32_t tile, int32_t zchunk);
 int lbm_get_kernel(lbm_ctx* ctx, int32_t* variant, int32_t* tile, int32_t* zchunk);
 
@@ -205,7 +213,7 @@ int lbm_tile_shape(int32_t tile, int32_t* tx, int32_t* ty, int32_t* wrap);
  * of the fused kernel on this box's current state, reps launches each, into
  * a scratch level (the state is not advanced or changed), keeps the fastest
  * and returns it with its mean launch time in milliseconds.  Call after
- * lbm_init_fields (and lbm_set_particles). */
+ * lbm_init_fields (and lbm_set_particles).  LBM_ERR_ARG for a diffdens box. */
 int lbm_autotune(lbm_ctx* ctx, int32_t reps, int32_t* tile, int32_t* zchunk, double* best_ms);
 
 /* Name of the collide-stream kernel variant in use and its algorithmic bytes
@@ -255,7 +263,7 @@ int lbm_set_closure(lbm_ctx* ctx, const float* block);
 
 /* Ensemble of independent boxes (no reference counterpart: the reference
  * runs one box per process).  Boxes may differ in size, particles and
- * parameters but share the storage format.  Each box is an ordinary lbm_ctx
+ * parameters but share the storage format (and are not diffdens boxes).  Each box is an ordinary lbm_ctx
  * (initialise, set particles and read results through the calls above);
  * lbm_ensemble_step advances all of them together: per step one persistent
  * launch runs the fluid of every box from a shared work list of

@@ -36,7 +36,8 @@ class LbmConfig(ctypes.Structure):
         ("lubrication", ctypes.c_int32),
         ("lub_k", ctypes.c_float), ("lub_rmin", ctypes.c_float), ("lub_rcap", ctypes.c_float),
         ("closure", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("diffdens", ctypes.c_int32), ("dens_r", ctypes.c_float), ("dens_b", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -56,6 +57,7 @@ def default_config(n, **kw):
     c.hertz, c.hz_k, c.hz_rmin, c.hz_rcap = 1, 10.0, 10.5, 9.0
     c.lubrication, c.lub_k, c.lub_rmin, c.lub_rcap = 0, 10.0, 10.0, 9.0
     c.closure = 0
+    c.diffdens, c.dens_r, c.dens_b = 0, 1.0, 1.0
     for k, v in kw.items():
         if k in ("ext_force", "ext_torque"):
             arr = getattr(c, k)

@@ -132,7 +132,8 @@ void build_ring(const std::vector<int8_t>& tmpl, std::vector<int8_t>& ring) {
 
 /* default tile for the fused kernel: the whole-row tile when one fits */
 void default_kernel(lbm_ctx* c) {
-  c->variant = LBM_KERNEL_FUSED;
+  c->variant = c->cfg.diffdens ? LBM_KERNEL_TWOPASS : LBM_KERNEL_FUSED;   /* the density-ratio model is
+                                                                             two-kernel only */
   c->tile = 0;
   const int prefer[] = {5, 7, 9};   /* 256x2, 128x4, 64x4 with x wrap */
   for (int t : prefer)
@@ -174,6 +175,7 @@ void lbm_config_default(lbm_config* cfg, int32_t n) {
   cfg->hertz = 1;
   cfg->hz_k = 10.0f; cfg->hz_rmin = 10.5f; cfg->hz_rcap = 9.0f;
   cfg->lub_k = 10.0f; cfg->lub_rmin = 10.0f; cfg->lub_rcap = 9.0f;
+  cfg->dens_r = cfg->dens_b = 1.0f;
 }
 
 int lbm_fill_random_rho(uint64_t seed, double mean, double amp, int32_t nx, int32_t ny, int32_t nz,
@@ -205,6 +207,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
   /* shifted 16-bit storage: the fused kernel re-stores non-fluid values
      through float, exact while w_0 rho0 + |f - w rho0| < 0.5 */
   if (cfg->storage != LBM_STORE_F32 && !(cfg->store_rho0 >= 0.0f && cfg->store_rho0 <= 1.2f)) return LBM_ERR_ARG;
+  /* DIFFDENS: "the density of at least one of the two fluids should be one"
+     (SetConstants, main.CUF:1058-1062) */
+  if (cfg->diffdens && !(cfg->dens_r > 0.0f && cfg->dens_b > 0.0f && (cfg->dens_r == 1.0f || cfg->dens_b == 1.0f)))
+    return LBM_ERR_ARG;
   /* template offsets are int8: radius + 3 must stay below 127 */
   if (cfg->with_particles && (cfg->radius < 1.0f || cfg->radius > 60.0f)) return LBM_ERR_ARG;
   /* particle kernels wrap offsets of up to rmax + 2 nodes once per axis */
@@ -783,7 +789,7 @@ int lbm_set_kernel(lbm_ctx* c, int32_t variant, int32_t tile, int32_t zchunk) {
     c->variant = variant;
     return LBM_OK;
   }
-  if (variant != LBM_KERNEL_FUSED) return LBM_ERR_ARG;
+  if (variant != LBM_KERNEL_FUSED || c->cfg.diffdens) return LBM_ERR_ARG;
   if (zchunk <= 0) zchunk = c->nz < 32 ? c->nz : 32;
   if (zchunk > c->nz) zchunk = c->nz;
   if (!lbm_fused_tile_valid(c, tile, zchunk)) return LBM_ERR_ARG;
@@ -814,7 +820,7 @@ int lbm_tile_shape(int32_t tile, int32_t* tx, int32_t* ty, int32_t* wrap) {
 }
 
 int lbm_autotune(lbm_ctx* c, int32_t reps, int32_t* tile_out, int32_t* zc_out, double* best_ms) {
-  if (!c || reps <= 0) return LBM_ERR_ARG;
+  if (!c || reps <= 0 || c->cfg.diffdens) return LBM_ERR_ARG;
   if (!c->initialised) return LBM_ERR_STATE;
   /* the fused kernel only reads the current level, so repeated launches into
      a scratch level leave the state as it was; the status word is restored */
@@ -986,7 +992,7 @@ int lbm_ensemble_create(int32_t nbox, const lbm_config* cfgs, lbm_ensemble** out
   if (!out || !cfgs || nbox <= 0) return LBM_ERR_ARG;
   *out = nullptr;
   for (int i = 0; i < nbox; ++i)
-    if (cfgs[i].storage != cfgs[0].storage) return LBM_ERR_BATCH;
+    if (cfgs[i].storage != cfgs[0].storage || cfgs[i].diffdens) return LBM_ERR_BATCH;
   lbm_ensemble* e = new (std::nothrow) lbm_ensemble();
   if (!e) return LBM_ERR_ALLOC;
   e->fb = nullptr;

@@ -49,6 +49,14 @@ __host__ __device__ constexpr float BL(int l) { return l == 0 ? B0 : (l < 7 ? B1
 constexpr float INV_SQRT2 = 0.70710678118654752440f;
 __host__ __device__ constexpr float INV_REC(int l) { return l == 0 ? 0.0f : (l < 7 ? 1.0f : INV_SQRT2); }
 
+/* density-ratio model tables phi, varphi, psi, xi (dimensions_m.CUF:230-262):
+ * equilCG = rho (phi_l + varphi_l alpha_k + p_l (u-terms)); psi_l and xi_l
+ * weigh the v.grad(rho_k) and diagonal (v grad rho_k + transpose) terms */
+__host__ __device__ constexpr float PHI_DD(int l) { return l == 0 ? 0.0f : (l < 7 ? 1.0f / 12.0f : 1.0f / 24.0f); }
+__host__ __device__ constexpr float VARPHI_DD(int l) { return l == 0 ? 1.0f : (l < 7 ? -1.0f / 12.0f : -1.0f / 24.0f); }
+__host__ __device__ constexpr float PSI_DD(int l) { return l == 0 ? -2.5f : (l < 7 ? -1.0f / 6.0f : 1.0f / 24.0f); }
+__host__ __device__ constexpr float XI_DD(int l) { return l == 0 ? 0.0f : (l < 7 ? 0.25f : 0.125f); }
+
 /* D3Q27 gradient weights ad3q27 = pd3q27/cssq, dimensions_m.CUF:283-300 */
 constexpr float P1D = (2.0f / 3.0f) * (2.0f / 3.0f) * (1.0f / 6.0f);
 constexpr float P2D = (1.0f / 6.0f) * (1.0f / 6.0f) * (2.0f / 3.0f);
@@ -196,6 +204,10 @@ struct FluidArgs {
   float min_rho, max_rho;
   DevStatus* status;
   int step;
+  /* density-ratio model (diffdens != 0): densR, densB, alphaCG(1:2) */
+  int diffdens;
+  float dens_r, dens_b;
+  float alpha_r, alpha_b;
 };
 
 }  // namespace lbm

@@ -43,6 +43,13 @@ __device__ __forceinline__ float equil(float rho, float u, float v, float w, int
   return rho * W(l) * (1.0f + uv + 0.5f * (uv * uv) - (0.5f / CSSQ) * (u * u + v * v + w * w));
 }
 
+/* equilCG, kernels_fluid_CG.CUF:40-49: colour k's rest weight alpha_k */
+__device__ __forceinline__ float equil_cg(float rho, float u, float v, float w, float alpha, int l) {
+  float uv = (1.0f / CSSQ) * (u * (float)EX(l) + v * (float)EY(l) + w * (float)EZ(l));
+  return rho * (PHI_DD(l) + VARPHI_DD(l) * alpha +
+                W(l) * (uv + 0.5f * (uv * uv) - (0.5f / CSSQ) * (u * u + v * v + w * w)));
+}
+
 /* Initial equilibrium state: setup2 / setup2_zplanes / setupCyl / setupPops
  * (kernels_fluid_CG.CUF:140-346).  For host-supplied densities the rho arrays
  * are filled before the launch. */
@@ -67,6 +74,10 @@ __global__ __launch_bounds__(256) void k_init_eq(FluidArgs a, int type, float vx
     float t = fcut(rd, radius, radius + 0.1f);
     rr = t;
     rb = 1.0f - t;
+    if (a.diffdens && type == LBM_INIT_DROPLET) {   /* setup2, :163-169 */
+      rr = t * a.dens_r;
+      rb = (1.0f - t) * a.dens_b;
+    }
     a.rho_r[n] = rr;
     a.rho_b[n] = rb;
   } else if (type == LBM_INIT_ZPLANES) {
@@ -86,9 +97,12 @@ __global__ __launch_bounds__(256) void k_init_eq(FluidArgs a, int type, float vx
   }
   void* dst = a.dst;                /* current level */
   void* oth = (void*)a.src;         /* other level: cleared */
+  const bool dd = a.diffdens && type == LBM_INIT_DROPLET;   /* equilCG in setup2 only, :176-182 */
 #pragma unroll
   for (int l = 0; l < 19; ++l) {
-    st_pair<S>(dst, a.N, l, n, equil(rr, u, v, w, l), equil(rb, u, v, w, l), a.rho0);
+    const float er = dd ? equil_cg(rr, u, v, w, a.alpha_r, l) : equil(rr, u, v, w, l);
+    const float eb = dd ? equil_cg(rb, u, v, w, a.alpha_b, l) : equil(rb, u, v, w, l);
+    st_pair<S>(dst, a.N, l, n, er, eb, a.rho0);
     st_pair<S>(oth, a.N, l, n, 0.0f, 0.0f, a.rho0);
   }
   fl0[n] = FL_FLUID;
@@ -135,7 +149,7 @@ __global__ __launch_bounds__(256) void k_density(FluidArgs a) {
  *                   except the slots a fluid neighbour claims for bounce-back;
  *   dead nodes    : only the rest population is carried (flipflop*Pop0_CG).
  */
-template <class S, bool PART>
+template <class S, bool PART, bool DD>
 __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
   const int x = blockIdx.x * BX + threadIdx.x;
   const int y = blockIdx.y * BY + threadIdx.y;
@@ -161,7 +175,8 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
     for (int l = 0; l < 19; ++l) ld_pair<S>(src, N, l, n, a.rho0, fr[l], fb[l]);
     const float rR = a.rho_r[n], rB = a.rho_b[n];
     const float rs = rR + rB;
-    const float phf = a.phi[n];
+    /* DIFFDENS: the phase field of normalised densities rho_k/dens_k, :691-692 */
+    const float phf = DD ? (rR / a.dens_r - rB / a.dens_b) / (rR / a.dens_r + rB / a.dens_b) : a.phi[n];
     const float visc = 1.0f / (0.5f * (1.0f + phf) * a.inv_visc_r + 0.5f * (1.0f - phf) * a.inv_visc_b);
     const float omega = 1.0f / (visc / CSSQ + 0.5f);
     const float omm = 1.0f - omega;
@@ -175,22 +190,52 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
     /* 26-point gradient in the D3Q27 order of kernels_fluid_CG.CUF:720-741;
        zero-component terms add nothing and are skipped */
     float px = 0.0f, py = 0.0f, pz = 0.0f;
+    float gR[3] = {0.0f, 0.0f, 0.0f}, gB[3] = {0.0f, 0.0f, 0.0f};   /* psi_R, psi_B (DIFFDENS) */
 #pragma unroll
     for (int l = 1; l < 27; ++l) {
       const int dx = EX27(l), dy = EY27(l), dz = EZ27(l);
-      const float ps = a.phi[nb(dx, dy, dz)];
+      float ps;
+      if (DD) {
+        const size_t m = nb(dx, dy, dz);
+        const float ra = a.rho_r[m], rb = a.rho_b[m];
+        ps = (ra / a.dens_r - rb / a.dens_b) / (ra / a.dens_r + rb / a.dens_b);
+        if (dx) { gR[0] = gR[0] + A27(l) * (float)dx * ra; gB[0] = gB[0] + A27(l) * (float)dx * rb; }
+        if (dy) { gR[1] = gR[1] + A27(l) * (float)dy * ra; gB[1] = gB[1] + A27(l) * (float)dy * rb; }
+        if (dz) { gR[2] = gR[2] + A27(l) * (float)dz * ra; gB[2] = gB[2] + A27(l) * (float)dz * rb; }
+      } else {
+        ps = a.phi[nb(dx, dy, dz)];
+      }
       if (dx) px = px + A27(l) * (float)dx * ps;
       if (dy) py = py + A27(l) * (float)dy * ps;
       if (dz) pz = pz + A27(l) * (float)dz * ps;
     }
     /* BGK towards f^eq(rho_k, u) per colour, :794-800 */
-    const float usq = (0.5f / CSSQ) * (u * u + v * v + w * w);
+    if (DD) {
+      /* DIFFDENS BGK (:767-792): equilCG plus visc (psi_l v.psi_k + xi_l
+         sum_i G_k(i,i) e_li^2) with G_k = v (x) psi_k + its transpose */
+      const float vR = u * gR[0] + v * gR[1] + w * gR[2];
+      const float vB = u * gB[0] + v * gB[1] + w * gB[2];
+      const float mR0 = u * gR[0], mR1 = v * gR[1], mR2 = w * gR[2];
+      const float mB0 = u * gB[0], mB1 = v * gB[1], mB2 = w * gB[2];
+      const float GR0 = mR0 + mR0, GR1 = mR1 + mR1, GR2 = mR2 + mR2;
+      const float GB0 = mB0 + mB0, GB1 = mB1 + mB1, GB2 = mB2 + mB2;
 #pragma unroll
-    for (int l = 0; l < 19; ++l) {
-      const float uv = (1.0f / CSSQ) * (u * (float)EX(l) + v * (float)EY(l) + w * (float)EZ(l));
-      const float brk = 1.0f + uv + 0.5f * (uv * uv) - usq;
-      fr[l] = fr[l] * omm + rR * W(l) * brk * omega;
-      fb[l] = fb[l] * omm + rB * W(l) * brk * omega;
+      for (int l = 0; l < 19; ++l) {
+        const float cx = (float)(EX(l) * EX(l)), cy = (float)(EY(l) * EY(l)), cz = (float)(EZ(l) * EZ(l));
+        const float tR = GR0 * cx + GR1 * cy + GR2 * cz;
+        const float tB = GB0 * cx + GB1 * cy + GB2 * cz;
+        fr[l] = fr[l] * omm + (equil_cg(rR, u, v, w, a.alpha_r, l) + visc * (PSI_DD(l) * vR + XI_DD(l) * tR)) * omega;
+        fb[l] = fb[l] * omm + (equil_cg(rB, u, v, w, a.alpha_b, l) + visc * (PSI_DD(l) * vB + XI_DD(l) * tB)) * omega;
+      }
+    } else {
+      const float usq = (0.5f / CSSQ) * (u * u + v * v + w * w);
+#pragma unroll
+      for (int l = 0; l < 19; ++l) {
+        const float uv = (1.0f / CSSQ) * (u * (float)EX(l) + v * (float)EY(l) + w * (float)EZ(l));
+        const float brk = 1.0f + uv + 0.5f * (uv * uv) - usq;
+        fr[l] = fr[l] * omm + rR * W(l) * brk * omega;
+        fb[l] = fb[l] * omm + rB * W(l) * brk * omega;
+      }
     }
     const float pn2 = px * px + py * py + pz * pz;
     if (pn2 > 1.e-20f) {
@@ -207,7 +252,8 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
         const float fd = fr[l] + fb[l] + ac * t;
         const float cph = ed * INV_REC(l) * ipn;
         const float tt = bcoef * cph;
-        const float feq = rR * W(l) + rB * W(l);
+        const float feq = DD ? equil_cg(rR, 0.0f, 0.0f, 0.0f, a.alpha_r, l) + equil_cg(rB, 0.0f, 0.0f, 0.0f, a.alpha_b, l)
+                             : rR * W(l) + rB * W(l);
         fr[l] = fd * rR * inv + tt * feq;
         fb[l] = fd * rB * inv - tt * feq;
       }
@@ -411,6 +457,24 @@ FluidArgs make_args(lbm_ctx* c) {
   a.min_rho = c->cfg.min_rho; a.max_rho = c->cfg.max_rho;
   a.status = c->d_status;
   a.step = (int)c->step;
+  a.diffdens = c->cfg.diffdens;
+  a.dens_r = 1.0f;
+  a.dens_b = 1.0f;
+  a.alpha_r = a.alpha_b = W0;
+  if (c->cfg.diffdens) {
+    /* SetConstants, main.CUF:1047-1085 (the else branch keeps the
+       reference's expression as written) */
+    const float dR = c->cfg.dens_r, dB = c->cfg.dens_b;
+    a.dens_r = dR;
+    a.dens_b = dB;
+    if (dR == 1.0f) {
+      a.alpha_r = W0;
+      a.alpha_b = (-1.0f + a.alpha_r) * (dR / dB) + 1.0f;
+    } else {
+      a.alpha_b = W0;
+      a.alpha_r = (1.0f / (-1.0f + a.alpha_b)) * (dB / dR) + 1.0f;
+    }
+  }
   return a;
 }
 
@@ -449,8 +513,13 @@ int lbm_launch_collide(lbm_ctx* c) {
   bool part = c->np > 0;
   lbm_timing_begin(c);
   LBM_STORAGE_SWITCH(c->storage,
-    if (part) hipLaunchKernelGGL((k_collide_push<S, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_collide_push<S, false>), grid, block, 0, c->stream, a))
+    if (a.diffdens) {
+      if (part) hipLaunchKernelGGL((k_collide_push<S, true, true>), grid, block, 0, c->stream, a);
+      else hipLaunchKernelGGL((k_collide_push<S, false, true>), grid, block, 0, c->stream, a);
+    } else {
+      if (part) hipLaunchKernelGGL((k_collide_push<S, true, false>), grid, block, 0, c->stream, a);
+      else hipLaunchKernelGGL((k_collide_push<S, false, false>), grid, block, 0, c->stream, a);
+    })
   lbm_timing_end(c);
   return hipGetLastError() == hipSuccess ? 0 : LBM_ERR_HIP;
 }

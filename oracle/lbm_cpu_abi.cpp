@@ -37,6 +37,7 @@ void lbm_config_default(lbm_config* cfg, int32_t n) {
   cfg->hertz = 1;
   cfg->hz_k = 10.0f; cfg->hz_rmin = 10.5f; cfg->hz_rcap = 9.0f;
   cfg->lub_k = 10.0f; cfg->lub_rmin = 10.0f; cfg->lub_rcap = 9.0f;
+  cfg->dens_r = cfg->dens_b = 1.0f;
 }
 
 int lbm_create(const lbm_config* cfg, lbm_ctx** out) {

@@ -32,6 +32,15 @@
  * The lubrication branch is applied to the FP64 sums myf2 with a viscosity
  * from tau (the reference adds it non-atomically to forceAtoms, which
  * addForce_particle_bounce_back then overwrites, PART:1463-1501, 1280-1282).
+ *
+ * The density-ratio model (config diffdens: the reference's DIFFDENS build,
+ * equilCG kernels_fluid_CG.CUF:40-49, psi/xi terms :755-792, alphaCG
+ * main.CUF:1077-1083, setup2 :163-182) is pinned by
+ * tests/test_oracle_diffdens.py: equilibrium moments against the closed forms
+ * of a colour-dependent sound speed, pressure balance of the pure phases,
+ * mass and momentum conservation of the correction terms.  The alphaCG
+ * branch for densR != 1 is the reference's expression as written and is
+ * parity unpinned (no fact fixes it; it does not balance the pressures).
  */
 #include "../include/lbcuda_amd.h"
 #include "lbm_oracle.h"
@@ -77,6 +86,9 @@ struct Oracle {
   /* constants (SetConstants, main.CUF:1043-1162) */
   R cssq, p[19], bl[19], ad27[27], pd27[27], rec_fact[19], buz;
   R viscR, viscB, densR, densB, sigma, beta;
+  /* DIFFDENS (dimensions_m.CUF:230-262, main.CUF:1047-1085) */
+  bool dd = false;
+  R alphaCG[2], phi_t[19], varphi_t[19], psi_t[19], xi_t[19];
 
   std::vector<R> pops[2];   /* colour 0 = R, 1 = B; (x,y,z,l,flip) */
   std::vector<R> rho[2];
@@ -139,6 +151,29 @@ struct Oracle {
       rec_fact[l] = std::sqrt(R(EX[l] * EX[l] + EY[l] * EY[l] + EZ[l] * EZ[l]));
     buz = R(2) / cssq;
     densR = R(1); densB = R(1);
+    dd = c->diffdens != 0;
+    if (dd) {
+      if (!(c->dens_r > 0 && c->dens_b > 0 && (c->dens_r == 1.0f || c->dens_b == 1.0f))) return LBM_ERR_ARG;
+      densR = R(c->dens_r);
+      densB = R(c->dens_b);
+    }
+    /* alphaCG: equal pressures rho_R (1 - alpha_R) = rho_B (1 - alpha_B) when
+       densR = 1; the other branch is the reference's expression as written */
+    if (densR == R(1)) {
+      alphaCG[0] = p0;
+      alphaCG[1] = (-R(1) + alphaCG[0]) * (densR / densB) + R(1);
+    } else {
+      alphaCG[1] = p0;
+      alphaCG[0] = (R(1) / (-R(1) + alphaCG[1])) * (densB / densR) + R(1);
+    }
+    for (int l = 0; l < 19; ++l) {
+      const int g = l == 0 ? 0 : (l < 7 ? 1 : 2);
+      const R ph[3] = {R(0), R(1) / R(12), R(1) / R(24)};
+      const R vp[3] = {R(1), -R(1) / R(12), -R(1) / R(24)};
+      const R ps[3] = {-R(5) / R(2), -R(1) / R(6), R(1) / R(24)};
+      const R xs[3] = {R(0), R(1) / R(4), R(1) / R(8)};
+      phi_t[l] = ph[g]; varphi_t[l] = vp[g]; psi_t[l] = ps[g]; xi_t[l] = xs[g];
+    }
     viscR = cssq * (R(c->tau_r) - R(0.5));
     viscB = cssq * (R(c->tau_b) - R(0.5));
     sigma = R(c->sigma);
@@ -266,10 +301,17 @@ struct Oracle {
     return R(0.5) * std::cos((r - r1) * R(3.141592653589793238462643383279502884) / (r2 - r1)) + R(0.5);
   }
 
-  void set_node_eq(long i, long j, long k, R rR, R rB, R u, R v, R w) {
+  /* equilCG, kernels_fluid_CG.CUF:40-49 (colour c = 0 red, 1 blue) */
+  R equil_cg(R rh, R u, R v, R w, int c, int l) const {
+    R uv = (R(1) / cssq) * (u * R(EX[l]) + v * R(EY[l]) + w * R(EZ[l]));
+    return rh * (phi_t[l] + varphi_t[l] * alphaCG[c] +
+                 p[l] * (uv + R(0.5) * (uv * uv) - (R(0.5) / cssq) * (u * u + v * v + w * w)));
+  }
+
+  void set_node_eq(long i, long j, long k, R rR, R rB, R u, R v, R w, bool cg = false) {
     for (int l = 0; l < 19; ++l) {
-      P(0, 1, l, i, j, k) = equil(rR, u, v, w, l);
-      P(1, 1, l, i, j, k) = equil(rB, u, v, w, l);
+      P(0, 1, l, i, j, k) = cg ? equil_cg(rR, u, v, w, 0, l) : equil(rR, u, v, w, l);
+      P(1, 1, l, i, j, k) = cg ? equil_cg(rB, u, v, w, 1, l) : equil(rB, u, v, w, l);
       P(0, 2, l, i, j, k) = R(0);
       P(1, 2, l, i, j, k) = R(0);
     }
@@ -304,6 +346,7 @@ struct Oracle {
             R rdist = std::sqrt(dx * dx + dy * dy + dz * dz);
             R t = fcut(rdist, radius, radius + R(0.1));
             rR = t; rB = R(1) - t;
+            if (dd) { rR = t * densR; rB = (R(1) - t) * densB; }   /* setup2, :163-169 */
           } else if (type == LBM_INIT_ZPLANES) { /* setup2_zplanes */
             R t = (glz * 1 / 8 < k && k < glz * 5 / 8) ? R(0) : R(1);
             rR = t; rB = R(1) - t;
@@ -323,7 +366,7 @@ struct Oracle {
           }
           RH(0, i, j, k) = rR;
           RH(1, i, j, k) = rB;
-          set_node_eq(i, j, k, rR, rB, u, v, w);
+          set_node_eq(i, j, k, rR, rB, u, v, w, dd && type == LBM_INIT_DROPLET);
         }
     initialised = true;
     if (natoms > 0) init_particles_step0();
@@ -745,6 +788,7 @@ struct Oracle {
           auto mw = [&](const R* f) { return inv * (f[5] - f[6] + f[11] - f[12] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18]); };
           R u = mu(tR) + mu(tB), v = mv(tR) + mv(tB), w = mw(tR) + mw(tB);
           R px = 0, py = 0, pz = 0;
+          R gR[3] = {0, 0, 0}, gB[3] = {0, 0, 0};   /* psi_R, psi_B (DIFFDENS, :755-762) */
           for (int l = 1; l < 27; ++l) {
             long i1 = i + EX27[l], j1 = j + EY27[l], k1 = k + EZ27[l];
             R a = RH(0, i1, j1, k1), b = RH(1, i1, j1, k1);
@@ -752,9 +796,33 @@ struct Oracle {
             px = px + ad27[l] * R(EX27[l]) * ps;
             py = py + ad27[l] * R(EY27[l]) * ps;
             pz = pz + ad27[l] * R(EZ27[l]) * ps;
+            if (dd) {
+              gR[0] = gR[0] + ad27[l] * R(EX27[l]) * a; gR[1] = gR[1] + ad27[l] * R(EY27[l]) * a;
+              gR[2] = gR[2] + ad27[l] * R(EZ27[l]) * a;
+              gB[0] = gB[0] + ad27[l] * R(EX27[l]) * b; gB[1] = gB[1] + ad27[l] * R(EY27[l]) * b;
+              gB[2] = gB[2] + ad27[l] * R(EZ27[l]) * b;
+            }
           }
-          for (int l = 0; l < 19; ++l) tR[l] = tR[l] * omm + equil(rR, u, v, w, l) * omega;
-          for (int l = 0; l < 19; ++l) tB[l] = tB[l] * omm + equil(rB, u, v, w, l) * omega;
+          if (dd) {
+            /* :767-792: G = v (x) psi + transpose; only its diagonal meets
+               cmat(i,i,l) = e_li^2 */
+            const R vel[3] = {u, v, w};
+            const R* gs[2] = {gR, gB};
+            R* ts[2] = {tR, tB};
+            const R rk[2] = {rR, rB};
+            for (int c = 0; c < 2; ++c) {
+              R vdot = vel[0] * gs[c][0] + vel[1] * gs[c][1] + vel[2] * gs[c][2];
+              R G[3];
+              for (int d = 0; d < 3; ++d) G[d] = vel[d] * gs[c][d] + vel[d] * gs[c][d];
+              for (int l = 0; l < 19; ++l) {
+                R tens = G[0] * R(EX[l] * EX[l]) + G[1] * R(EY[l] * EY[l]) + G[2] * R(EZ[l] * EZ[l]);
+                ts[c][l] = ts[c][l] * omm + (equil_cg(rk[c], u, v, w, c, l) + visc * (psi_t[l] * vdot + xi_t[l] * tens)) * omega;
+              }
+            }
+          } else {
+            for (int l = 0; l < 19; ++l) tR[l] = tR[l] * omm + equil(rR, u, v, w, l) * omega;
+            for (int l = 0; l < 19; ++l) tB[l] = tB[l] * omm + equil(rB, u, v, w, l) * omega;
+          }
           R pn2 = px * px + py * py + pz * pz;
           if (pn2 > mylimit) {
             R pn = std::sqrt(pn2);
@@ -765,7 +833,8 @@ struct Oracle {
               fd[l] = tR[l] + tB[l] + ac * t;
             }
             for (int l = 0; l < 19; ++l) {
-              R feq = equil(rR, 0, 0, 0, l) + equil(rB, 0, 0, 0, l);
+              R feq = dd ? equil_cg(rR, 0, 0, 0, 0, l) + equil_cg(rB, 0, 0, 0, 1, l)
+                         : equil(rR, 0, 0, 0, l) + equil(rB, 0, 0, 0, l);
               R ed = R(EX[l]) * px + R(EY[l]) * py + R(EZ[l]) * pz;
               R t1 = rec_fact[l] * pn;
               R cph = (t1 <= mylimit) ? R(0) : ed / t1;
