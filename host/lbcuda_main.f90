@@ -8,7 +8,10 @@
 ! optional namelist file lbcuda.nml:
 !   &lbcuda box=256, taur=1.0, taub=1.0, sigma=0.1, beta=0.999, storage='f32',
 !           wetting='janus', radius=5.5, rmass=6.6667e-4, lubrication=.false.,
-!           seed=1234, autotune=.false., restart_every=0 /
+!           seed=1234, autotune=.false., restart_every=0,
+!           diffdens=.false., densr=1.0, densb=1.0 /
+! (diffdens selects the density-ratio model of the reference's DIFFDENS build,
+! whose densR/densB the reference leaves to be set in SetConstants.)
 ! Output follows OutputVTK (main.CUF:1812-1929): at step 0 and every nIterVTK
 ! steps, rhoB ('rho2'), phase, rho ('rho1'), vel and the spheres go to
 ! output/ in the reference's formats.  restart_every > 0 writes the initType 4
@@ -29,7 +32,8 @@ program lbcuda_host
   logical :: withParticles, lrotate, lubrication, lexist
   character(len=120) :: string
   character(len=8) :: storage, wetting
-  real :: taur, taub, sigma, beta, radius, rmass
+  real :: taur, taub, sigma, beta, radius, rmass, densr, densb
+  logical :: diffdens
   type(lbm_config) :: cfg
   type(c_ptr) :: ctx
   integer(c_int) :: rc
@@ -47,12 +51,13 @@ program lbcuda_host
   integer(c_int32_t) :: tile, zchunk
   real(c_double) :: best_ms
   namelist /lbcuda/ box, taur, taub, sigma, beta, storage, wetting, radius, rmass, lubrication, seed, autotune, &
-                     restart_every
+                     restart_every, diffdens, densr, densb
 
   ! defaults: dimensions_m.CUF:7 (MYDIMESION 64), :50, :59-60, :103-104
   box = 64; taur = 1.0; taub = 1.0; sigma = 0.1; beta = 0.999
   storage = 'f32'; wetting = 'janus'; radius = 5.5; rmass = 1.0/1500.0
   lubrication = .false.; seed = 1234; autotune = .false.; restart_every = 0
+  diffdens = .false.; densr = 1.0; densb = 1.0
 
   inquire(file='lbcuda.nml', exist=lexist)
   if (lexist) then
@@ -128,6 +133,9 @@ program lbcuda_host
   ! ---- configuration ----
   call lbm_config_default(cfg, int(box, c_int32_t))
   cfg%tau_r = taur; cfg%tau_b = taub; cfg%sigma = sigma; cfg%beta = beta
+  if (diffdens) then
+    cfg%diffdens = 1; cfg%dens_r = densr; cfg%dens_b = densb
+  end if
   select case (trim(storage))
   case ('f16');  cfg%storage = LBM_STORE_F16
   case ('bf16'); cfg%storage = LBM_STORE_BF16

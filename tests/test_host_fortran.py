@@ -347,3 +347,21 @@ def test_gpu_host_restart_from_checkpoint_is_exact(tmp_path):
     a = open(os.path.join(d1, "output", "particle_000012.vtk")).read()
     b = open(os.path.join(d2, "output", "particle_000006.vtk")).read()
     assert a == b
+
+
+def test_diffdens_deck(cpu_host, tmp_path):
+    """diffdens in lbcuda.nml selects the density-ratio model: the droplet
+    start scales the blue density by densB (setup2, kernels_fluid_CG.CUF:163-169)
+    and both colour masses are conserved."""
+    d = str(tmp_path / "dd")
+    write_case(d, DROPLET, nml="&lbcuda box=16, sigma=0.05, beta=0.7, diffdens=.true., densb=4.0 /\n")
+    r = run(cpu_host, d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m0 = [float(x) for x in r.stdout.split("initial mass R,B:")[1].split()[:2]]
+    m1 = [float(x) for x in r.stdout.split("final mass R,B:")[1].split()[:2]]
+    d0 = str(tmp_path / "plain")
+    write_case(d0, DROPLET, nml="&lbcuda box=16, sigma=0.05, beta=0.7 /\n")
+    r0 = run(cpu_host, d0)
+    p0 = [float(x) for x in r0.stdout.split("initial mass R,B:")[1].split()[:2]]
+    assert m0[0] == p0[0] and abs(m0[1] - 4.0 * p0[1]) < 1e-9 * m0[1]
+    assert np.allclose(m0, m1, rtol=1e-10)
