@@ -257,6 +257,12 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     c->nball = (int)(ball.size() / 4);
     c->nring = (int)(ring.size() / 4);
     c->ntlist = (int)(tlist.size() / 4);
+    /* the boxed shell pass indexes a (2 rmax + 1)^3 LDS box by these offsets */
+    c->shell_box_ok = true;
+    for (const std::vector<int8_t>* tb : {&tmpl, &ring})
+      for (size_t t = 0; t < tb->size() / 4; ++t)
+        for (int d = 0; d < 3; ++d)
+          if (std::abs((int)(*tb)[4 * t + d]) > c->rmax) c->shell_box_ok = false;
     rc |= dalloc(&c->d_tmpl, tmpl.size());
     rc |= dalloc(&c->d_ring, ring.size());
     rc |= dalloc(&c->d_tlist, tlist.size());

@@ -291,6 +291,7 @@ struct lbm_ctx {
   int8_t* d_ball;         /* (4, nball) offsets with r^2 <= rmax^2; [3]=1 if <   */
   int nball;
   int rmax;
+  bool shell_box_ok;      /* template and ring offsets within rmax (LDS shell pass) */
   /* near-contact closure: parameters + normalisation (LBM_CLOSURE_NFLOAT) */
   float* d_closure;       /* NULL when disabled                                */
   float closure_gap_max;
@@ -310,6 +311,7 @@ struct lbm_ctx {
 /* batched (ensemble) step: opaque per-phase state, lbm_fused.hip / lbm_particles.hip */
 struct lbm_fluid_batch;
 struct lbm_part_batch;
+size_t lbm_particles_shell_lds(const lbm_ctx* c);
 lbm_fluid_batch* lbm_fluid_batch_create(lbm_ctx* const* boxes, int nbox, int* rc);
 void lbm_fluid_batch_destroy(lbm_fluid_batch* b);
 bool lbm_fluid_batch_tile_ok(int tile);

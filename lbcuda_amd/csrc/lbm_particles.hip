@@ -476,8 +476,8 @@ __device__ __forceinline__ void surface_node(const PartArgs& a, int p, const int
 #pragma unroll
     for (int l = 1; l < 27; ++l) {
       if (f27[l] == FL_FLUID) {
-        Rs = Rs + PD27(l) * r27[l];
-        Bs = Bs + PD27(l) * b27[l];
+        Rs = fmaf(PD27(l), r27[l], Rs);   /* fused in both shell passes alike */
+        Bs = fmaf(PD27(l), b27[l], Bs);
         Ds = Ds + PD27(l);
       }
     }
@@ -585,6 +585,154 @@ __device__ __forceinline__ void k_shellwet_body(const PartArgs& a, int p) {
   }
   __syncthreads();
   for (int u = threadIdx.x; u < a.ntlist; u += TPB) surface_node<S>(a, p, c, u);
+}
+
+/* ---------------- shell pass with the particle's neighbourhood in LDS ----
+ * The same work as k_shellwet_body, but the types and densities the surface
+ * pass reads around each surface node (26 D3Q27 neighbours, all of them in
+ * the particle's template or ring) come from a box of side 2 rmax + 1 in LDS
+ * that the template and ring loop fill as they go, instead of being gathered
+ * again from global memory: template nodes are non-fluid (their type is not
+ * needed beyond that), ring nodes carry their type and, if fluid, the
+ * densities this pass forms (or make_fluid's, for a node filled this step). */
+struct ShellBox {
+  float* rr;
+  float* rb;
+  uint8_t* fl;
+  int h, w;
+  __device__ __forceinline__ int at(int dx, int dy, int dz) const { return (dx + h) + w * ((dy + h) + w * (dz + h)); }
+};
+
+__device__ __forceinline__ ShellBox shell_box(const PartArgs& a) {
+  extern __shared__ float shell_lds[];
+  ShellBox b;
+  b.h = a.rmax;
+  b.w = 2 * a.rmax + 1;
+  const int n = b.w * b.w * b.w;
+  b.rr = shell_lds;
+  b.rb = shell_lds + n;
+  b.fl = reinterpret_cast<uint8_t*>(shell_lds + 2 * n);
+  return b;
+}
+
+__device__ __forceinline__ void shell_zero_box(const PartArgs& a, int p, const int c[3], uint32_t w, const ShellBox& bx) {
+  bx.fl[bx.at(off_c(w, 0), off_c(w, 1), off_c(w, 2))] = FL_DEAD;
+  shell_zero(a, p, c, w);
+}
+
+template <class S>
+__device__ __forceinline__ void shell_ring_box(const PartArgs& a, const int c[3], uint32_t w, const ShellBox& bx) {
+  const size_t n = node_at(a, c[0], c[1], c[2], off_c(w, 0), off_c(w, 1), off_c(w, 2));
+  const uint8_t f = a.fl_cur[n], fo = a.fl_old[n];
+  float rr, rb;
+  rho_sum<S>(a.pop_src, a.N, n, a.rho0, &rr, &rb);   /* issued with the type loads */
+  const int b = bx.at(off_c(w, 0), off_c(w, 1), off_c(w, 2));
+  bx.fl[b] = f;
+  if (f != FL_FLUID) return;
+  if (fo != FL_FLUID) {
+    rr = a.rho_r[n];   /* filled this step: make_fluid's densities */
+    rb = a.rho_b[n];
+  } else {
+    a.rho_r[n] = rr;
+    a.rho_b[n] = rb;
+    a.phi[n] = (rr - rb) / (rr + rb);
+  }
+  bx.rr[b] = rr;
+  bx.rb[b] = rb;
+}
+
+/* surface_node with the neighbours' types and densities from the box */
+template <class S>
+__device__ __forceinline__ void surface_node_box(const PartArgs& a, int p, const int c[3], int t, const ShellBox& bx) {
+  const int ox = a.tlist[4 * t], oy = a.tlist[4 * t + 1], oz = a.tlist[4 * t + 2];
+  int gx, gy, gz;
+  size_t n = node_at(a, c[0], c[1], c[2], ox, oy, oz, &gx, &gy, &gz);
+  uint32_t* mslot = a.smask + (size_t)p * a.ntlist + t;
+  const uint8_t fn = a.fl_cur[n];
+  const uint32_t kn = a.key_cur[n];
+  if ((fn != FL_LIST) | ((kn >> 3) != (uint32_t)p)) {
+    *mslot = 0u;
+    return;
+  }
+  uint32_t mask = 1u << 31;
+#pragma unroll
+  for (int l = 1; l < 19; ++l)
+    if (bx.fl[bx.at(ox + EX(l), oy + EY(l), oz + EZ(l))] == FL_FLUID) mask |= 1u << (l - 1);
+  *mslot = mask;
+  if (a.wetting == LBM_WET_NONE) return;
+  float rr, rb;
+  if (a.wetting == LBM_WET_JANUS) {
+    /* compute_virtual_rho, PART:299-352: sign of r.(q x q*) */
+    double r[3] = {(double)(gx + 1) - a.pos_cur[13 * p], (double)(gy + 1) - a.pos_cur[13 * p + 1],
+                   (double)(gz + 1) - a.pos_cur[13 * p + 2]};
+    r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
+    r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
+    r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
+    const double q0 = a.pos_cur[13 * p + 9], q1 = a.pos_cur[13 * p + 10], q2 = a.pos_cur[13 * p + 11],
+                 q3 = a.pos_cur[13 * p + 12];
+    const double ax = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    const double ay = 2.0 * (q1 * q2 + q0 * q3);
+    const double az = 2.0 * (q1 * q3 - q0 * q2);
+    const double ac = ax * r[0] + ay * r[1] + az * r[2];
+    if (ac < 0.0) { rr = 0.0f; rb = 1.0f; } else { rr = 1.0f; rb = 0.0f; }
+  } else {
+    /* compute_densities_wall + compute_onebelt_density_2fl_weights, PART:267-297, 864-894 */
+    float Rs = 0.0f, Bs = 0.0f, Ds = 0.0f;
+#pragma unroll
+    for (int l = 1; l < 27; ++l) {
+      const int b = bx.at(ox + EX27(l), oy + EY27(l), oz + EZ27(l));
+      if (bx.fl[b] == FL_FLUID) {
+        Rs = fmaf(PD27(l), bx.rr[b], Rs);
+        Bs = fmaf(PD27(l), bx.rb[b], Bs);
+        Ds = Ds + PD27(l);
+      }
+    }
+    if (Ds != 0.0f) {
+      rr = Rs / Ds;
+      rb = Bs / Ds;
+    } else {
+      float R2 = 0.0f, B2 = 0.0f;
+      int D = 0;
+      for (int l = 0; l < 98; ++l) {
+        size_t m = wrap_node(a, gx + C_BELT2[l][0], gy + C_BELT2[l][1], gz + C_BELT2[l][2]);
+        if (a.fl_cur[m] == FL_FLUID && a.fl_old[m] == FL_FLUID) {
+          float mr, mb;
+          belt_rho<S>(a, m, &mr, &mb);
+          R2 = R2 + mr;
+          B2 = B2 + mb;
+          D = D + 1;
+        }
+      }
+      if (D == 0) return;   /* fix_onebelt_density_twofluids is empty */
+      rr = R2 / (float)D;
+      rb = B2 / (float)D;
+    }
+  }
+  a.rho_r[n] = rr;
+  a.rho_b[n] = rb;
+  a.phi[n] = (rr - rb) / (rr + rb);
+}
+
+template <class S>
+__device__ __forceinline__ void k_shellbox_body(const PartArgs& a, int p) {
+  k_neigh_body(a, p);
+  k_fill_release_body(a, p);
+  const ShellBox bx = shell_box(a);
+  int c[3];
+  centre(a.pos_cur, p, c);
+  const int nmax = max(a.ntmpl, a.nring);
+  int t = threadIdx.x;
+  uint32_t wt = t < a.ntmpl ? off_word(a.tmpl, t) : 0u, wr = t < a.nring ? off_word(a.ring, t) : 0u;
+  for (; t < nmax; t += TPB) {
+    const int tn = t + TPB;
+    const uint32_t wtn = tn < a.ntmpl ? off_word(a.tmpl, tn) : 0u, wrn = tn < a.nring ? off_word(a.ring, tn) : 0u;
+    if (t < a.ntmpl) shell_zero_box(a, p, c, wt, bx);
+    if (t < a.nring) shell_ring_box<S>(a, c, wr, bx);
+    wt = wtn;
+    wr = wrn;
+  }
+  __syncthreads();
+  for (int u = threadIdx.x; u < a.ntlist; u += TPB) surface_node_box<S>(a, p, c, u, bx);
 }
 
 /* ---------------- bounce-back on surface links ---------------------------
@@ -1120,6 +1268,7 @@ LBM_PART_KERNEL_S(k_erase)
 LBM_PART_KERNEL_S(k_fill_apply)
 LBM_PART_KERNEL_S(k_wetting)
 LBM_PART_KERNEL_S(k_shellwet)
+LBM_PART_KERNEL_S(k_shellbox)
 LBM_PART_KERNEL_S(k_links)
 
 /* pair forces of particle i, then (thread 0) its rigid-body update: the
@@ -1248,8 +1397,23 @@ int lbm_particles_wetting(lbm_ctx* c) {
 
 int lbm_particles_shell_wetting(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
-  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shellwet<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  const size_t lds = lbm_particles_shell_lds(c);
+  if (lds) {
+    LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shellbox<S>, dim3(c->np), dim3(TPB), lds, c->stream, a))
+  } else {
+    LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shellwet<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  }
   return last_err();
+}
+
+/* LDS bytes of the boxed shell pass, 0 when the box is too large (then the
+ * pass gathers from global memory): every template and ring offset must lie
+ * within rmax of the centre */
+size_t lbm_particles_shell_lds(const lbm_ctx* c) {
+  if (!c->shell_box_ok) return 0;
+  const size_t w = 2 * (size_t)c->rmax + 1, n = w * w * w;
+  const size_t bytes = ((2 * sizeof(float) + 1) * n + 15) & ~(size_t)15;
+  return bytes <= 24 * 1024 ? bytes : 0;
 }
 
 int lbm_particles_bounce(lbm_ctx* c) {
