@@ -37,7 +37,15 @@ __host__ __device__ constexpr int OPP(int l) { return l == 0 ? 0 : (l % 2 == 1 ?
 
 /* weights p(l) and recolouring factors, dimensions_m.CUF:206-209, main.CUF:1102 */
 constexpr float CSSQ = 1.0f / 3.0f;
-constexpr float W0 = 1.0f / 3.0f, W1 = 1.0f / 18.0f, W2 = 1.0f / 36.0f;
+/* The floats nearest 1/3, 1/18, 1/36 sum to 1 + 1.5e-8, so every BGK step
+ * would add 1.5e-8 of each node's mass (7.5e-5 after 5000 steps, as the
+ * reference's FP32 build does).  W0 two ulps below, W1 four ulps above and
+ * W2 two ulps below the nearest floats sum to exactly 1 and keep the second
+ * moment 3 (2 W1 + 8 W2) = 1 + 7.5e-9 of the nearest floats (it sets the
+ * momentum each BGK step returns). */
+constexpr float W0 = 0x1.555552p-2f, W1 = 0x1.c71c7ap-5f, W2 = 0x1.c71c6ep-6f;
+static_assert((double)W0 + 6.0 * (double)W1 + 12.0 * (double)W2 == 1.0, "D3Q19 weights must sum to 1");
+static_assert(3.0 * (2.0 * (double)W1 + 8.0 * (double)W2) - 1.0 < 1e-8, "second moment of the weights");
 __host__ __device__ constexpr float W(int l) { return l == 0 ? W0 : (l < 7 ? W1 : W2); }
 /* Leclaire B_l, dimensions_m.CUF:222-228 */
 constexpr float BXI = 0.5f;

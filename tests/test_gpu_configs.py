@@ -59,17 +59,16 @@ def test_c1_taylor_green_64_1000_steps():
         rate = -math.log(eg / e0) / t
         print(f"  step {t}: KE gpu/oracle - 1 = {eg / eo - 1.0:.2e}, decay rate / 4 nu k^2 - 1 = "
               f"{rate / (4.0 * nu * k * k) - 1.0:.2e}", flush=True)
-        # FP32 against FP64: absolute velocity errors of ~1e-8 grow relative to
-        # the decaying energy (-7e-5 at step 750)
-        assert abs(eg / eo - 1.0) < 1e-3, (t, eg, eo)
+        # FP32 against FP64: the energy decays by e^-6.4 over the run, and the
+        # float weights (exact sum, second moment 1 + 7.5e-9) shift the decay
+        # rate by 2e-5 of itself; the difference reaches 9e-4 at step 1000
+        assert abs(eg / eo - 1.0) < 2e-3, (t, eg, eo)
         # exact Navier-Stokes decay rate; the oracle's lattice rate is 0.08 %
         # above it at k = 2 pi/64 (higher-order lattice viscosity terms)
         assert abs(rate / (4.0 * nu * k * k) - 1.0) < 2e-3, t
     fg, fo = g.pops(0), o.pops(0)
     ug = (fg * EX).sum(-1) / fg.sum(-1)
     uo = (fo * EX).sum(-1) / fo.sum(-1)
-    # 4.8e-6 and 2.0e-7 measured: FP32 rounding plus the 1.5e-8 per step mass
-    # bias of float weights (see the spinodal case)
     assert np.abs(fg - fo).max() < 1e-5 and np.abs(ug - uo).max() < 5e-7
     g.close()
     o.close()
@@ -139,10 +138,10 @@ def test_c2_spinodal_128_5000_steps(spinodal_128):
         drift[storage] = dm
         # the mixture has separated: most nodes are in one phase
         assert interface < 0.5
-    # FP32 weights 1/3, 1/18, 1/36 sum to 1 + 1.5e-8, so each BGK step adds
-    # 1.5e-8 of the mass (as the reference's FP32 build does): 7.5e-5 after
-    # 5000 steps; the FP64 oracle conserves to 1e-13
-    assert drift[STORE_F32] < 1.5e-4
+    # FP32: the float weights sum to exactly 1 (the nearest floats to 1/3,
+    # 1/18, 1/36 would add 1.5e-8 of the mass per step, 7.5e-5 here, as the
+    # reference's FP32 build does); 3.8e-7 measured
+    assert drift[STORE_F32] < 5e-6
     # 16-bit storage: rounding noise makes a colour's density dip below zero
     # where the other phase fills the node, and the MINDENS clamp of the
     # density pass (kernels_fluid_CG.CUF:443-446) turns that noise into a mass
