@@ -178,7 +178,9 @@ int lbm_get_sums(lbm_ctx* ctx, double out[5]);
  * same count), after which lbm_step continues bit for bit as the saved box
  * would have.  The closure block (lbm_set_closure) is configuration and is
  * not part of the image.  LBM_ERR_ARG for a foreign or truncated image or a
- * different config, LBM_ERR_STATE for a different particle count. */
+ * different config, LBM_ERR_STATE for a different particle count (both
+ * checked before anything is copied); after a HIP error during the copy the
+ * box state is undefined until the next successful load or init. */
 int lbm_checkpoint_size(lbm_ctx* ctx, int64_t* bytes);
 int lbm_checkpoint_save(lbm_ctx* ctx, void* buf, int64_t bytes);
 int lbm_checkpoint_load(lbm_ctx* ctx, const void* buf, int64_t bytes);
