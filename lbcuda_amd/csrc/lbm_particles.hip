@@ -25,6 +25,10 @@ using namespace lbm;
 namespace {
 
 constexpr int TPB = 256;
+/* wider workgroups for the passes that walk a particle's whole ball or ring
+ * (fewer iterations per thread on their dependent gathers) */
+constexpr int NT_SHELL = 512;
+constexpr int NT_BALL = 512;
 constexpr int MAXNB = 32;   /* neighbour-list capacity per particle */
 
 /* D3Q27 weights pd3q27 */
@@ -120,9 +124,11 @@ __device__ bool moved(const PartArgs& a, int p) {   /* particle_moved, PART:363-
   return c1[0] != c2[0] || c1[1] != c2[1] || c1[2] != c2[2];
 }
 
-/* workgroup FP64 reduction of 6 values; result valid in thread 0 */
+/* workgroup FP64 reduction of 6 values over NT threads; result valid in
+ * thread 0 */
+template <int NT = TPB>
 __device__ void block_reduce6(double v[6]) {
-  __shared__ double sh[6][TPB / 64];
+  __shared__ double sh[6][NT / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
@@ -135,7 +141,7 @@ __device__ void block_reduce6(double v[6]) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       double s = 0.0;
-      for (int w = 0; w < TPB / 64; ++w) s += sh[q][w];
+      for (int w = 0; w < NT / 64; ++w) s += sh[q][w];
       v[q] = s;
     }
   __syncthreads();
@@ -254,13 +260,13 @@ __device__ __forceinline__ void k_resolve_body(const PartArgs& a, int p) {
 /* ---------------- erase: fluid -> solid (del_fluid, PART:383-561) --------
  * The same pass over the old ball claims the nodes that turn fluid for
  * make_fluid (k_fill_apply). */
-template <class S>
+template <class S, int NT>
 __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+  for (int t = threadIdx.x; t < a.nball; t += NT) {
     int gx, gy, gz;
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
     const uint8_t fnew = a.fl_cur[n], fold = a.fl_old[n];
@@ -297,7 +303,7 @@ __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
       acc[3] += tq[0]; acc[4] += tq[1]; acc[5] += tq[2];
     }
   }
-  block_reduce6(acc);
+  block_reduce6<NT>(acc);
   if (threadIdx.x == 0) {
     for (int d = 0; d < 3; ++d) a.myf2[3 * p + d] += acc[d];
     if (a.lrotate)
@@ -311,14 +317,14 @@ __device__ bool fill_candidate(const PartArgs& a, size_t n) {
   return (fo == FL_LIST || fo == FL_DEAD) && a.fl_cur[n] == FL_FLUID;
 }
 
-template <class S>
+template <class S, int NT>
 __device__ __forceinline__ void k_fill_apply_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const float vx = (float)a.pos_cur[13 * p + 3], vy = (float)a.pos_cur[13 * p + 4], vz = (float)a.pos_cur[13 * p + 5];
-  for (int t = threadIdx.x; t < a.nball; t += TPB) {
+  for (int t = threadIdx.x; t < a.nball; t += NT) {
     int gx, gy, gz;
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2], &gx, &gy, &gz);
     if (!fill_candidate(a, n)) continue;
@@ -396,7 +402,7 @@ __device__ __forceinline__ void k_fill_apply_body(const PartArgs& a, int p) {
       acc[3] += tq[0]; acc[4] += tq[1]; acc[5] += tq[2];
     }
   }
-  block_reduce6(acc);
+  block_reduce6<NT>(acc);
   if (threadIdx.x == 0) {
     for (int d = 0; d < 3; ++d) a.myf2[3 * p + d] += acc[d];
     if (a.lrotate)
@@ -404,11 +410,12 @@ __device__ __forceinline__ void k_fill_apply_body(const PartArgs& a, int p) {
   }
 }
 
+template <int NT = TPB>
 __device__ __forceinline__ void k_fill_release_body(const PartArgs& a, int p) {
   if (!moved(a, p)) return;
   int co[3];
   centre(a.pos_old, p, co);
-  for (int t = blockIdx.y * TPB + threadIdx.x; t < a.nball; t += TPB * gridDim.y) {
+  for (int t = blockIdx.y * NT + threadIdx.x; t < a.nball; t += NT * gridDim.y) {
     size_t n = node_at(a, co[0], co[1], co[2], a.ball[4 * t], a.ball[4 * t + 1], a.ball[4 * t + 2]);
     a.claim[n] = -1;
   }
@@ -555,6 +562,7 @@ __device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], ui
   a.phi[n] = (rr - rb) / (rr + rb);
 }
 
+template <int NT = TPB>
 __device__ __forceinline__ void k_neigh_body(const PartArgs& a, int i);
 
 /* shell densities, wetting and the link masks of the surface nodes in one
@@ -713,18 +721,18 @@ __device__ __forceinline__ void surface_node_box(const PartArgs& a, int p, const
   a.phi[n] = (rr - rb) / (rr + rb);
 }
 
-template <class S>
+template <class S, int NT>
 __device__ __forceinline__ void k_shellbox_body(const PartArgs& a, int p) {
-  k_neigh_body(a, p);
-  k_fill_release_body(a, p);
+  k_neigh_body<NT>(a, p);
+  k_fill_release_body<NT>(a, p);
   const ShellBox bx = shell_box(a);
   int c[3];
   centre(a.pos_cur, p, c);
   const int nmax = max(a.ntmpl, a.nring);
   int t = threadIdx.x;
   uint32_t wt = t < a.ntmpl ? off_word(a.tmpl, t) : 0u, wr = t < a.nring ? off_word(a.ring, t) : 0u;
-  for (; t < nmax; t += TPB) {
-    const int tn = t + TPB;
+  for (; t < nmax; t += NT) {
+    const int tn = t + NT;
     const uint32_t wtn = tn < a.ntmpl ? off_word(a.tmpl, tn) : 0u, wrn = tn < a.nring ? off_word(a.ring, tn) : 0u;
     if (t < a.ntmpl) shell_zero_box(a, p, c, wt, bx);
     if (t < a.nring) shell_ring_box<S>(a, c, wr, bx);
@@ -732,7 +740,7 @@ __device__ __forceinline__ void k_shellbox_body(const PartArgs& a, int p) {
     wr = wrn;
   }
   __syncthreads();
-  for (int u = threadIdx.x; u < a.ntlist; u += TPB) surface_node_box<S>(a, p, c, u, bx);
+  for (int u = threadIdx.x; u < a.ntlist; u += NT) surface_node_box<S>(a, p, c, u, bx);
 }
 
 /* ---------------- bounce-back on surface links ---------------------------
@@ -969,16 +977,17 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
  * order), so k_links finds every particle whose ball reaches i's surface.
  * More than MAXNB raises LBM_ERR_LIST_OVERFLOW, as makeListAtomsGPU raises
  * on a full tile list (kernels_fluid_PART.CUF:2436-2439). */
+template <int NT>
 __device__ __forceinline__ void k_neigh_body(const PartArgs& a, int i) {
   int ci[3];
   centre(a.pos_cur, i, ci);
-  __shared__ int wcount[TPB / 64];
+  __shared__ int wcount[NT / 64];
   __shared__ int base;
   if (threadIdx.x == 0) base = 0;
   __syncthreads();
   const int lim = 4 * a.rmax * a.rmax;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < a.np; j0 += TPB) {
+  for (int j0 = 0; j0 < a.np; j0 += NT) {
     const int j = j0 + threadIdx.x;
     bool hit = false;
     if (j < a.np) {
@@ -1003,7 +1012,7 @@ __device__ __forceinline__ void k_neigh_body(const PartArgs& a, int i) {
     }
     __syncthreads();
     if (threadIdx.x == 0)
-      for (int w = 0; w < TPB / 64; ++w) base += wcount[w];
+      for (int w = 0; w < NT / 64; ++w) base += wcount[w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -1259,16 +1268,26 @@ __device__ __forceinline__ PartArgs batch_args(const PartArgs* __restrict__ args
     NAME##_body<S>(batch_args(args, it.x, b), it.y);                                                  \
   }
 
+#define LBM_PART_KERNEL_SN(NAME, NT)                                                                  \
+  template <class S>                                                                                  \
+  __global__ __launch_bounds__(NT) void NAME(PartArgs a) { NAME##_body<S, NT>(a, blockIdx.x); }    \
+  template <class S>                                                                                  \
+  __global__ __launch_bounds__(NT) void NAME##_batch(const PartArgs* __restrict__ args,            \
+                                                     const int2* __restrict__ items, BatchStep b) {  \
+    const int2 it = items[blockIdx.x];                                                                \
+    NAME##_body<S, NT>(batch_args(args, it.x, b), it.y);                                              \
+  }
+
 LBM_PART_KERNEL(k_unstamp)
 LBM_PART_KERNEL(k_stamp)
 LBM_PART_KERNEL(k_resolve)
 LBM_PART_KERNEL(k_fill_release)
 LBM_PART_KERNEL(k_neigh)
-LBM_PART_KERNEL_S(k_erase)
-LBM_PART_KERNEL_S(k_fill_apply)
+LBM_PART_KERNEL_SN(k_erase, NT_BALL)
+LBM_PART_KERNEL_SN(k_fill_apply, NT_BALL)
 LBM_PART_KERNEL_S(k_wetting)
 LBM_PART_KERNEL_S(k_shellwet)
-LBM_PART_KERNEL_S(k_shellbox)
+LBM_PART_KERNEL_SN(k_shellbox, NT_SHELL)
 LBM_PART_KERNEL_S(k_links)
 
 /* pair forces of particle i, then (thread 0) its rigid-body update: the
@@ -1373,7 +1392,7 @@ int lbm_particles_flags(lbm_ctx* c, int level) {
 
 int lbm_particles_erase(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
-  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_erase<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_erase<S>, dim3(c->np), dim3(NT_BALL), 0, c->stream, a))
   return last_err();
 }
 
@@ -1382,7 +1401,7 @@ int lbm_particles_erase(lbm_ctx* c) {
 int lbm_particles_fill(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
   const dim3 g((unsigned)c->np, lbm_particles_chunks(c->ntmpl, c->nball));
-  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_fill_apply<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
+  LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_fill_apply<S>, dim3(c->np), dim3(NT_BALL), 0, c->stream, a))
   if (c->variant != LBM_KERNEL_FUSED) hipLaunchKernelGGL(k_fill_release, g, dim3(TPB), 0, c->stream, a);
   return last_err();
 }
@@ -1399,7 +1418,7 @@ int lbm_particles_shell_wetting(lbm_ctx* c) {
   PartArgs a = make_pargs(c);
   const size_t lds = lbm_particles_shell_lds(c);
   if (lds) {
-    LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shellbox<S>, dim3(c->np), dim3(TPB), lds, c->stream, a))
+    LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shellbox<S>, dim3(c->np), dim3(NT_SHELL), lds, c->stream, a))
   } else {
     LBM_STORAGE_SWITCH(c->storage, hipLaunchKernelGGL(k_shellwet<S>, dim3(c->np), dim3(TPB), 0, c->stream, a))
   }
@@ -1538,13 +1557,13 @@ int lbm_part_batch_pre(lbm_part_batch* b, int parity, long step, hipStream_t st)
   BatchStep bs;
   bs.step = (int)step;
   bs.lag_from_pops = step > 1 ? 1 : 0;
-  const dim3 g((unsigned)b->nitems), gc((unsigned)b->nitems, b->chunks), t(TPB);
+  const dim3 g((unsigned)b->nitems), gc((unsigned)b->nitems, b->chunks), t(TPB), tb(NT_BALL);
   hipLaunchKernelGGL(k_unstamp_batch, gc, t, 0, st, A, b->d_items, bs);
   hipLaunchKernelGGL(k_stamp_batch, gc, t, 0, st, A, b->d_items, bs);
   hipLaunchKernelGGL(k_resolve_batch, gc, t, 0, st, A, b->d_items, bs);
   LBM_STORAGE_SWITCH(b->storage,
-    hipLaunchKernelGGL(k_erase_batch<S>, g, t, 0, st, A, b->d_items, bs);
-    hipLaunchKernelGGL(k_fill_apply_batch<S>, g, t, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_erase_batch<S>, g, tb, 0, st, A, b->d_items, bs);
+    hipLaunchKernelGGL(k_fill_apply_batch<S>, g, tb, 0, st, A, b->d_items, bs);
     hipLaunchKernelGGL(k_shellwet_batch<S>, g, t, 0, st, A, b->d_items, bs))
   return last_err();
 }
