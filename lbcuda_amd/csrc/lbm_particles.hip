@@ -96,6 +96,17 @@ struct PartArgs {
 
 __device__ __forceinline__ long nint(double x) { return lround(x); }   /* Fortran NINT */
 
+/* x - n nint(x / n), the minimum image of an offset |x| < 1.5 n in a periodic
+ * box of n nodes, without the division: nint(x / n) is 1 exactly when
+ * x >= n/2 and -1 exactly when x <= -n/2, since the correctly rounded
+ * quotient of a double by an integer that small cannot cross 1/2 (the
+ * largest double below n/2 divided by n stays below the largest double
+ * below 1/2) */
+__device__ __forceinline__ double min_image(double x, int n) {
+  const double h = 0.5 * (double)n;
+  return x >= h ? x - (double)n : (x <= -h ? x + (double)n : x);
+}
+
 __device__ __forceinline__ size_t node_at(const PartArgs& a, int cx, int cy, int cz, int dx, int dy, int dz,
                                           int* gx = nullptr, int* gy = nullptr, int* gz = nullptr) {
   /* centres are 1-based lattice coordinates, nodes 0-based; centres lie in
@@ -162,9 +173,9 @@ __device__ void lever(const PartArgs& a, const double* pos, int p, int gx, int g
   r[1] = (double)(gy + 1) - pos[13 * p + 1];
   r[2] = (double)(gz + 1) - pos[13 * p + 2];
   if (minimg) {
-    r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
-    r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
-    r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
+    r[0] = min_image(r[0], a.nx);
+    r[1] = min_image(r[1], a.ny);
+    r[2] = min_image(r[2], a.nz);
   }
   double s = (double)a.radius / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
   r[0] *= s; r[1] *= s; r[2] *= s;
@@ -466,9 +477,9 @@ __device__ __forceinline__ void surface_node(const PartArgs& a, int p, const int
     /* compute_virtual_rho, PART:299-352: sign of r.(q x q*) */
     double r[3] = {(double)(gx + 1) - a.pos_cur[13 * p], (double)(gy + 1) - a.pos_cur[13 * p + 1],
                    (double)(gz + 1) - a.pos_cur[13 * p + 2]};
-    r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
-    r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
-    r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
+    r[0] = min_image(r[0], a.nx);
+    r[1] = min_image(r[1], a.ny);
+    r[2] = min_image(r[2], a.nz);
     const double q0 = a.pos_cur[13 * p + 9], q1 = a.pos_cur[13 * p + 10], q2 = a.pos_cur[13 * p + 11],
                  q3 = a.pos_cur[13 * p + 12];
     /* body x axis in the lab frame: first column of the rotation matrix */
@@ -673,9 +684,9 @@ __device__ __forceinline__ void surface_node_box(const PartArgs& a, int p, const
     /* compute_virtual_rho, PART:299-352: sign of r.(q x q*) */
     double r[3] = {(double)(gx + 1) - a.pos_cur[13 * p], (double)(gy + 1) - a.pos_cur[13 * p + 1],
                    (double)(gz + 1) - a.pos_cur[13 * p + 2]};
-    r[0] -= (double)a.nx * (double)nint(r[0] / (double)a.nx);
-    r[1] -= (double)a.ny * (double)nint(r[1] / (double)a.ny);
-    r[2] -= (double)a.nz * (double)nint(r[2] / (double)a.nz);
+    r[0] = min_image(r[0], a.nx);
+    r[1] = min_image(r[1], a.ny);
+    r[2] = min_image(r[2], a.nz);
     const double q0 = a.pos_cur[13 * p + 9], q1 = a.pos_cur[13 * p + 10], q2 = a.pos_cur[13 * p + 11],
                  q3 = a.pos_cur[13 * p + 12];
     const double ax = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
@@ -1083,9 +1094,9 @@ __device__ __forceinline__ void k_pairs_body(const PartArgs& a, int i, const Pai
     if (j == i) continue;
     /* addHertzian, PART:1385-1503, with the pair vector from i to j */
     double xdf = a.pos_cur[13 * j] - xi, ydf = a.pos_cur[13 * j + 1] - yi, zdf = a.pos_cur[13 * j + 2] - zi;
-    xdf -= (double)a.nx * (double)nint(xdf / (double)a.nx);
-    ydf -= (double)a.ny * (double)nint(ydf / (double)a.ny);
-    zdf -= (double)a.nz * (double)nint(zdf / (double)a.nz);
+    xdf = min_image(xdf, a.nx);
+    ydf = min_image(ydf, a.ny);
+    zdf = min_image(zdf, a.nz);
     const double rsq = xdf * xdf + ydf * ydf + zdf * zdf;
     if (rsq > cutsq) continue;
     const double rrr = sqrt(rsq);
