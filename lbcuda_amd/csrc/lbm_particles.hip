@@ -874,8 +874,12 @@ template <class S>
 __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
   __shared__ NbrCache nc;
   __shared__ int self_slot;
+  __shared__ unsigned shared_slots;   /* slots of the other particles covering some surface node */
   const int nnb = min(a.nnbr[p], MAXNB);
-  if (threadIdx.x == 0) self_slot = -1;
+  if (threadIdx.x == 0) {
+    self_slot = -1;
+    shared_slots = 0u;
+  }
   __syncthreads();
   if ((int)threadIdx.x < nnb) {
     const int k = threadIdx.x;
@@ -898,7 +902,7 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
   if (kp < 0) return;
   const uint32_t* msk = a.smask + (size_t)p * a.ntlist;
   double own[6] = {0, 0, 0, 0, 0, 0};
-  bool shared_node = false;   /* some surface node of this thread is covered by another particle */
+  unsigned others = 0u;   /* slots of the other particles covering a surface node of this thread */
   for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
     const uint32_t m = msk[t];
     if (!(m >> 31)) continue;
@@ -909,7 +913,7 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
     unsigned cover = 0;   /* cache slots of the particles whose ball holds s */
     for (int k = 0; k < nnb; ++k)
       if (ball_r2(a, nc.c[k], gx, gy, gz) < r2max) cover |= 1u << k;
-    if (cover & ~(1u << kp)) shared_node = true;
+    others |= cover & ~(1u << kp);
     /* bounce-back: dst(ii, lin) = dst(s, lout) minus the moving-wall term of
        every covering particle, in ascending order; the owner's link momentum
        on the way */
@@ -950,12 +954,16 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
     if (a.lrotate)
       for (int d = 0; d < 3; ++d) a.myt[3 * p + d] = own[3 + d];
   }
-  /* shares of the other particles in contact with this one */
-  const bool any_shared = __syncthreads_or(shared_node);
+  /* shares of the other particles in contact with this one: a pass over the
+     surface for each particle whose ball covers one of its nodes, zero for
+     the rest of the list */
+  if (others) atomicOr(&shared_slots, others);
+  __syncthreads();
+  const unsigned sh = shared_slots;
   for (int k = 0; k < nnb; ++k) {
     if (k == kp) continue;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (any_shared) {
+    if (sh >> k & 1u) {
       for (int t = threadIdx.x; t < a.ntlist; t += TPB) {
         const uint32_t m = msk[t];
         if (!(m >> 31)) continue;
