@@ -338,6 +338,7 @@ void lbm_timing_begin(lbm_ctx* c);
 void lbm_timing_end(lbm_ctx* c);
 
 /* launchers implemented in lbm_fluid.hip / lbm_particles.hip */
+lbm::FluidArgs lbm_fluid_args(lbm_ctx* c);
 int lbm_launch_density(lbm_ctx* c);
 int lbm_launch_collide(lbm_ctx* c);
 int lbm_launch_fused(lbm_ctx* c);

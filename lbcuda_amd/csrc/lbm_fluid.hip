@@ -482,6 +482,9 @@ inline unsigned nblocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / 
 
 }  // namespace
 
+/* the fluid kernels' arguments of context c at its current step */
+FluidArgs lbm_fluid_args(lbm_ctx* c) { return make_args(c); }
+
 #define LBM_STORAGE_SWITCH(st, BODY)                 \
   switch (st) {                                      \
     case LBM_STORE_F32: { using S = float; BODY; } break;           \

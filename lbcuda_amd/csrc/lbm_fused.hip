@@ -496,25 +496,7 @@ constexpr TileEntry kTiles[] = {
 };
 constexpr int kNumTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
 
-FluidArgs make_fargs(lbm_ctx* c) {
-  FluidArgs a;
-  a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
-  a.N = c->N;
-  a.src = c->pop[c->cur];
-  a.dst = c->pop[1 - c->cur];
-  a.flags = c->flags[c->cur];
-  a.rho_r = c->rho_r; a.rho_b = c->rho_b; a.phi = c->phi;
-  const float viscR = CSSQ * (c->cfg.tau_r - 0.5f);
-  const float viscB = CSSQ * (c->cfg.tau_b - 0.5f);
-  a.inv_visc_r = 1.0f / viscR;
-  a.inv_visc_b = 1.0f / viscB;
-  a.sigma = c->cfg.sigma; a.beta = c->cfg.beta;
-  a.rho0 = c->storage == LBM_STORE_F32 ? 0.0f : c->cfg.store_rho0;
-  a.min_rho = c->cfg.min_rho; a.max_rho = c->cfg.max_rho;
-  a.status = c->d_status;
-  a.step = (int)c->step;
-  return a;
-}
+FluidArgs make_fargs(lbm_ctx* c) { return lbm_fluid_args(c); }
 
 template <class S, bool PART, int TX, int TY, bool WX>
 void launch_one(const FluidArgs& a, int zc, hipStream_t st) {
