@@ -32,6 +32,11 @@ extern "C" {
 #define LBM_STORE_F32  0
 #define LBM_STORE_F16  1   /* f_i - w_i*rho0 stored as IEEE half  */
 #define LBM_STORE_BF16 2   /* f_i - w_i*rho0 stored as bfloat16   */
+/* With either 16-bit storage the rest pair (i = 0) is a 16-bit float with 3
+ * exponent and 12 mantissa bits, |h| up to 2^(s-1) (1 - 2^-13), where the
+ * offset s is the smallest that holds twice the nominal colour density
+ * (s = 0, colour densities up to store_rho0 + 1.5, for the plain model);
+ * a rest value beyond it raises LBM_ERR_STORE_RANGE. */
 
 /* Wetting of particle surface nodes. */
 #define LBM_WET_NONE    0  /* surface rho stays MINDENS (init_rho*_CG)              */
@@ -69,6 +74,9 @@ extern "C" {
 #define LBM_ERR_LIST_OVERFLOW 11 /* more than 32 particles near one particle
                                     (the reference stops on >1000 particles per
                                     tile list, kernels_fluid_PART.CUF:2436-2439) */
+#define LBM_ERR_STORE_RANGE 12   /* a rest population beyond the range of
+                                    16-bit storage (no reference counterpart:
+                                    the reference stores float) */
 
 /*
  * Run-time configuration.  The reference fixes most of these at compile time

@@ -15,6 +15,7 @@ ERR_NAMES = {
     4: "LBM_ERR_RHO_RANGE", 5: "LBM_ERR_BIG_FORCE", 6: "LBM_ERR_EXTREME_VEL",
     7: "LBM_ERR_OUT_OF_BOX", 8: "LBM_ERR_PART_SETUP", 9: "LBM_ERR_STATE",
     10: "LBM_ERR_BATCH", 11: "LBM_ERR_LIST_OVERFLOW",
+    12: "LBM_ERR_STORE_RANGE",
 }
 
 

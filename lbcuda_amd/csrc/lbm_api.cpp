@@ -701,6 +701,7 @@ int lbm_status(lbm_ctx* c, int32_t* code, int32_t* step, char* msg, int32_t len)
     else if (st.code == LBM_ERR_EXTREME_VEL) m = "nve_lf] Extreme vel (kernels_fluid_PART.CUF:1545)";
     else if (st.code == LBM_ERR_OUT_OF_BOX) m = "nve] particle out of box (kernels_fluid_PART.CUF:1581)";
     else if (st.code == LBM_ERR_LIST_OVERFLOW) m = "particle neighbour list full (cf. kernels_fluid_PART.CUF:2436)";
+    else if (st.code == LBM_ERR_STORE_RANGE) m = "rest population beyond the range of 16-bit storage";
     std::snprintf(msg, (size_t)len, "%s", m);
   }
   return rc;

@@ -12,6 +12,8 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <algorithm>
+#include <cmath>
 
 #include "../../include/lbcuda_amd.h"
 
@@ -164,13 +166,79 @@ template <> struct Pair<__hip_bfloat16> {
   }
 };
 
-/* both colours of direction l at node n of a population level */
+/* The rest pair (l = 0) of 16-bit storage.  The shifted rest value
+ * h_0 = f_0 - w_0 rho0 is the largest stored deviation: about w_0 |rho_k -
+ * rho0| (0.017 in a 50/50 mixture, 0.17 where a colour is absent or at full
+ * density), where FP16 spacing is 2^-16 and 2^-13.  The rest pair is kept
+ * as two 16-bit floats with 3 exponent and 12 mantissa bits instead
+ * (sign | e | m): |h| = 2^(e - 9 + s) (1 + m / 4096) for e = 1..7 and
+ * m 2^(-20 + s) for e = 0, 4x finer than FP16 from 2^(-8 + s) up to the
+ * largest value, 2^(s - 1) (1 - 2^-13).  s comes from the configuration
+ * (rest_offset below); rounding is to nearest even, values beyond the range
+ * saturate (the collide passes report LBM_ERR_STORE_RANGE).
+ *
+ * The code of a value is the FP32 bit pattern of h 2^(-118 - s), whose
+ * exponent field is e (a denormal for e = 0), shifted right by 11 bits with
+ * rounding; the decoder shifts back and scales by 2^(118 + s), exact for
+ * every code (FP32 denormals are kept: denorm mode 3 in the kernel
+ * descriptors).  The two scales are rdec = 2^(118 + s), renc = 2^(-118 - s). */
+constexpr uint32_t REST_MAXCODE = 0x7fffu;
+__device__ __forceinline__ float rest_dec1(uint32_t code16, float rdec) {
+  const uint32_t bits = ((code16 & 0x7fffu) << 11) | ((code16 & 0x8000u) << 16);
+  return __uint_as_float(bits) * rdec;
+}
+__device__ __forceinline__ uint32_t rest_enc1(float h, float renc) {
+  const uint32_t u = __float_as_uint(h * renc);
+  const uint32_t mag = u & 0x7fffffffu;
+  const uint32_t code = min((mag + 0x3ffu + ((mag >> 11) & 1u)) >> 11, REST_MAXCODE);
+  return code | ((u >> 16) & 0x8000u);
+}
+__device__ __forceinline__ void rest_dec(uint32_t u, float rdec, float& r, float& b) {
+  r = rest_dec1(u & 0xffffu, rdec);
+  b = rest_dec1(u >> 16, rdec);
+}
+__device__ __forceinline__ uint32_t rest_enc(float r, float b, float renc) {
+  return rest_enc1(r, renc) | (rest_enc1(b, renc) << 16);
+}
+/* |h| rounds beyond the largest code: |h| 2^(-118 - s) at or above the
+ * midpoint between code 0x7fff and the next pattern, 2^-120 (2 - 2^-13) */
+__device__ __forceinline__ bool rest_out_of_range(float r, float b, float renc) {
+  return !(fmaxf(fabsf(r), fabsf(b)) * renc < 0x1.fff8p-120f);
+}
+
+/* Rest-pair exponent offset s of a configuration: the smallest s >= 0 whose
+ * range 2^(s - 1) holds a rest population of twice the nominal colour
+ * density: f_0 = w_0 rho_k with rho_k up to max(1, rho0) in the plain model
+ * (s = 0 stores colour densities up to rho0 + 1.5), f_0 up to alpha_k rho_k
+ * <= rho_k with rho_k up to max(densR, densB) in the density-ratio model. */
+inline int rest_offset(const lbm_config& cfg) {
+  const double rho0 = cfg.store_rho0;
+  const double fmax = cfg.diffdens ? 2.0 * std::max(1.0, (double)std::max(cfg.dens_r, cfg.dens_b))
+                                   : 2.0 * (double)W0 * std::max(1.0, rho0);
+  int s = 0;
+  while (s < 8 && std::ldexp(1.0, s - 1) < fmax - (double)W0 * rho0) ++s;
+  return s;
+}
+
+/* both colours of direction l at node n of a population level; rdec and
+ * renc are the rest-pair scales (16-bit storage) */
 template <class S>
-__device__ __forceinline__ void ld_pair(const void* pops, size_t N, int l, size_t n, float rho0, float& r, float& b) {
+__device__ __forceinline__ void ld_pair(const void* pops, size_t N, int l, size_t n, float rho0, float rdec, float& r,
+                                        float& b) {
+  if (sizeof(S) == 2 && l == 0) {
+    rest_dec(reinterpret_cast<const uint32_t*>(pops)[n], rdec, r, b);
+    r = r + W0 * rho0;
+    b = b + W0 * rho0;
+    return;
+  }
   Pair<S>::ld((const S*)pops, (size_t)l * N + n, W(l) * rho0, r, b);
 }
 template <class S>
-__device__ __forceinline__ void st_pair(void* pops, size_t N, int l, size_t n, float r, float b, float rho0) {
+__device__ __forceinline__ void st_pair(void* pops, size_t N, int l, size_t n, float r, float b, float rho0, float renc) {
+  if (sizeof(S) == 2 && l == 0) {
+    reinterpret_cast<uint32_t*>(pops)[n] = rest_enc(r - W0 * rho0, b - W0 * rho0, renc);
+    return;
+  }
   Pair<S>::st((S*)pops, (size_t)l * N + n, r, b, W(l) * rho0);
 }
 /* raw copy of one pair (no conversion) */
@@ -209,6 +277,7 @@ struct FluidArgs {
   float inv_visc_r, inv_visc_b;
   float sigma, beta;
   float rho0;            /* storage shift                                      */
+  float rdec, renc;      /* rest-pair decode / encode scales (16-bit storage)  */
   float min_rho, max_rho;
   DevStatus* status;
   int step;

@@ -102,8 +102,10 @@ __global__ __launch_bounds__(256) void k_init_eq(FluidArgs a, int type, float vx
   for (int l = 0; l < 19; ++l) {
     const float er = dd ? equil_cg(rr, u, v, w, a.alpha_r, l) : equil(rr, u, v, w, l);
     const float eb = dd ? equil_cg(rb, u, v, w, a.alpha_b, l) : equil(rb, u, v, w, l);
-    st_pair<S>(dst, a.N, l, n, er, eb, a.rho0);
-    st_pair<S>(oth, a.N, l, n, 0.0f, 0.0f, a.rho0);
+    if (sizeof(S) == 2 && l == 0 && rest_out_of_range(er - W0 * a.rho0, eb - W0 * a.rho0, a.renc))
+      raise_error(a.status, LBM_ERR_STORE_RANGE, a.step);
+    st_pair<S>(dst, a.N, l, n, er, eb, a.rho0, a.renc);
+    st_pair<S>(oth, a.N, l, n, 0.0f, 0.0f, a.rho0, a.renc);
   }
   fl0[n] = FL_FLUID;
   fl1[n] = FL_FLUID;
@@ -120,11 +122,11 @@ __global__ __launch_bounds__(256) void k_density(FluidArgs a) {
   if (n >= a.N) return;
   float rr = 0.0f, rb = 0.0f;
   if (a.flags[n] == FL_FLUID) {
-    ld_pair<S>(a.src, a.N, 0, n, a.rho0, rr, rb);
+    ld_pair<S>(a.src, a.N, 0, n, a.rho0, a.rdec, rr, rb);
 #pragma unroll
     for (int l = 1; l < 19; ++l) {
       float r, b;
-      ld_pair<S>(a.src, a.N, l, n, a.rho0, r, b);
+      ld_pair<S>(a.src, a.N, l, n, a.rho0, a.rdec, r, b);
       rr = rr + r;
       rb = rb + b;
     }
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
   if (f == FL_FLUID) {
     float fr[19], fb[19];
 #pragma unroll
-    for (int l = 0; l < 19; ++l) ld_pair<S>(src, N, l, n, a.rho0, fr[l], fb[l]);
+    for (int l = 0; l < 19; ++l) ld_pair<S>(src, N, l, n, a.rho0, a.rdec, fr[l], fb[l]);
     const float rR = a.rho_r[n], rB = a.rho_b[n];
     const float rs = rR + rB;
     /* DIFFDENS: the phase field of normalised densities rho_k/dens_k, :691-692 */
@@ -259,9 +261,11 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
       }
     }
     /* rest population stays, the others move to x + e_l */
-    st_pair<S>(dst, N, 0, n, fr[0], fb[0], a.rho0);
+    if (sizeof(S) == 2 && rest_out_of_range(fr[0] - W0 * a.rho0, fb[0] - W0 * a.rho0, a.renc))
+      raise_error(a.status, LBM_ERR_STORE_RANGE, a.step);
+    st_pair<S>(dst, N, 0, n, fr[0], fb[0], a.rho0, a.renc);
 #pragma unroll
-    for (int l = 1; l < 19; ++l) st_pair<S>(dst, N, l, nb(EX(l), EY(l), EZ(l)), fr[l], fb[l], a.rho0);
+    for (int l = 1; l < 19; ++l) st_pair<S>(dst, N, l, nb(EX(l), EY(l), EZ(l)), fr[l], fb[l], a.rho0, a.renc);
     if (PART) {
       /* bounce-back copy: surface node s = x + e_l whose far side s + e_l is
          not fluid sends f_l(x) back to x in direction opp(l) */
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(256) void k_collide_push(FluidArgs a) {
         if (a.flags[nb(EX(l), EY(l), EZ(l))] != FL_LIST) continue;
         const int x2 = wrapc(x + 2 * EX(l), nx), y2 = wrapc(y + 2 * EY(l), ny), z2 = wrapc(z + 2 * EZ(l), nz);
         if (a.flags[(size_t)x2 + (size_t)nx * (y2 + (size_t)ny * z2)] == FL_FLUID) continue;
-        st_pair<S>(dst, N, OPP(l), n, fr[l], fb[l], a.rho0);
+        st_pair<S>(dst, N, OPP(l), n, fr[l], fb[l], a.rho0, a.renc);
       }
     }
   } else {
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(256) void k_moments(FluidArgs a, float* rr_o, float
   if (n >= a.N) return;
   float fr[19], fb[19];
 #pragma unroll
-  for (int l = 0; l < 19; ++l) ld_pair<S>(a.src, a.N, l, n, a.rho0, fr[l], fb[l]);
+  for (int l = 0; l < 19; ++l) ld_pair<S>(a.src, a.N, l, n, a.rho0, a.rdec, fr[l], fb[l]);
   float lr = fr[0], lb = fb[0];
 #pragma unroll
   for (int l = 1; l < 19; ++l) { lr = lr + fr[l]; lb = lb + fb[l]; }
@@ -329,7 +333,7 @@ __global__ __launch_bounds__(256) void k_sums_partial(FluidArgs a, double* part)
 #pragma unroll
     for (int l = 0; l < 19; ++l) {
       float rf, bf;
-      ld_pair<S>(a.src, a.N, l, n, a.rho0, rf, bf);
+      ld_pair<S>(a.src, a.N, l, n, a.rho0, a.rdec, rf, bf);
       const double r = rf, b = bf;
       acc[0] += r;
       acc[1] += b;
@@ -374,11 +378,11 @@ __global__ __launch_bounds__(256) void k_rho_from_pops(FluidArgs a, const uint8_
   if (n >= a.N) return;
   float rr, rb;
   if (fl[n] == FL_FLUID) {
-    ld_pair<S>(a.src, a.N, 0, n, a.rho0, rr, rb);
+    ld_pair<S>(a.src, a.N, 0, n, a.rho0, a.rdec, rr, rb);
 #pragma unroll
     for (int l = 1; l < 19; ++l) {
       float r, b;
-      ld_pair<S>(a.src, a.N, l, n, a.rho0, r, b);
+      ld_pair<S>(a.src, a.N, l, n, a.rho0, a.rdec, r, b);
       rr = rr + r;
       rb = rb + b;
     }
@@ -397,7 +401,9 @@ __global__ __launch_bounds__(256) void k_unpack(FluidArgs a, int color, float* o
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N * 19) return;
   int l = (int)(i / a.N);
-  out[i] = Store<S>::ld((const S*)a.src, 2 * i + (size_t)color, W(l) * a.rho0);
+  float r, b;
+  ld_pair<S>(a.src, a.N, l, i - (size_t)l * a.N, a.rho0, a.rdec, r, b);
+  out[i] = color ? b : r;
 }
 
 template <class S>
@@ -405,7 +411,9 @@ __global__ __launch_bounds__(256) void k_pack(FluidArgs a, const float* pr, cons
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N * 19) return;
   int l = (int)(i / a.N);
-  Pair<S>::st((S*)a.dst, i, pr[i], pb[i], W(l) * a.rho0);
+  if (sizeof(S) == 2 && l == 0 && rest_out_of_range(pr[i] - W0 * a.rho0, pb[i] - W0 * a.rho0, a.renc))
+    raise_error(a.status, LBM_ERR_STORE_RANGE, a.step);
+  st_pair<S>(a.dst, a.N, l, i - (size_t)l * a.N, pr[i], pb[i], a.rho0, a.renc);
 }
 
 /* plain device copy (read n float4, write n float4): the measured HBM
@@ -454,6 +462,8 @@ FluidArgs make_args(lbm_ctx* c) {
   a.inv_visc_b = 1.0f / viscB;
   a.sigma = c->cfg.sigma; a.beta = c->cfg.beta;
   a.rho0 = c->storage == LBM_STORE_F32 ? 0.0f : c->cfg.store_rho0;
+  a.rdec = std::ldexp(1.0f, 118 + rest_offset(c->cfg));
+  a.renc = std::ldexp(1.0f, -118 - rest_offset(c->cfg));
   a.min_rho = c->cfg.min_rho; a.max_rho = c->cfg.max_rho;
   a.status = c->d_status;
   a.step = (int)c->step;

@@ -62,7 +62,9 @@ struct LevelIO {
   static constexpr uint32_t PS = 2 * sizeof(S);   /* bytes of one pair */
   Rsrc r[5];
   uint32_t pb;                                    /* bytes of one direction plane */
-  __device__ __forceinline__ LevelIO(const void* level, uint32_t plane_bytes) : pb(plane_bytes) {
+  float rdec, renc;                               /* rest-pair scales (16-bit)    */
+  __device__ __forceinline__ LevelIO(const void* level, uint32_t plane_bytes, float rest_dec_scale, float rest_enc_scale)
+      : pb(plane_bytes), rdec(rest_dec_scale), renc(rest_enc_scale) {
     char* base = const_cast<char*>(static_cast<const char*>(level));
 #pragma unroll
     for (int g = 0; g < 5; ++g)
@@ -76,12 +78,17 @@ struct LevelIO {
       return __builtin_bit_cast(typename Pair<S>::Raw,
                                 __builtin_amdgcn_raw_buffer_load_b32(r[l >> 2], node * PS, so, 0));
   }
+  /* a loaded pair of direction l as floats (no shift) */
+  __device__ __forceinline__ void cvt(int l, typename Pair<S>::Raw q, float& vr, float& vb) const;
   /* store a pair given as floats (already shifted for 16-bit storage); the
-   * 16-bit conversions round to nearest even, two values per instruction */
+   * 16-bit conversions round to nearest even, two values per instruction; the
+   * rest pair of 16-bit storage to its 3-exponent, 12-mantissa-bit format */
   __device__ __forceinline__ void st(int l, uint32_t node, float vr, float vb) const {
     const uint32_t so = (uint32_t)(l & 3) * pb;
     if constexpr (sizeof(S) == 4) {
       __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2U, make_float2(vr, vb)), r[l >> 2], node * PS, so, 0);
+    } else if (l == 0) {
+      __builtin_amdgcn_raw_buffer_store_b32(rest_enc(vr, vb, renc), r[0], node * PS, 0, 0);
     } else if constexpr (std::is_same<S, __half>::value) {
       typedef _Float16 H2 __attribute__((ext_vector_type(2)));
       const H2 h = {(_Float16)vr, (_Float16)vb};
@@ -107,6 +114,17 @@ __device__ __forceinline__ void cvt_raw(typename Pair<S>::Raw q, float& r, float
     r = __uint_as_float(q << 16);
     b = __uint_as_float(q & 0xffff0000u);
   }
+}
+
+template <class S>
+__device__ __forceinline__ void LevelIO<S>::cvt(int l, typename Pair<S>::Raw q, float& vr, float& vb) const {
+  if constexpr (sizeof(S) == 2) {
+    if (l == 0) {
+      rest_dec(__builtin_bit_cast(uint32_t, q), rdec, vr, vb);
+      return;
+    }
+  }
+  cvt_raw<S>(q, vr, vb);
 }
 
 /* reciprocal and reciprocal square root to 1 ulp (v_rcp_f32 / v_rsq_f32)
@@ -138,7 +156,7 @@ template <class S>
 __device__ __forceinline__ void load_sum(const LevelIO<S>& src, uint32_t n, float rho0, float (&fr)[19],
                                          float (&fb)[19], float& rr, float& rb) {
 #pragma unroll
-  for (int l = 0; l < 19; ++l) cvt_raw<S>(src.ld(l, n), fr[l], fb[l]);
+  for (int l = 0; l < 19; ++l) src.cvt(l, src.ld(l, n), fr[l], fb[l]);
   rr = fr[0];
   rb = fb[0];
 #pragma unroll
@@ -169,7 +187,7 @@ struct Ahead<S, false> {
   __device__ __forceinline__ void load(const LevelIO<S>& src, uint32_t n, float rho0, float& rr, float& rb) {
     load_sum<S>(src, n, rho0, r, b, rr, rb);
   }
-  __device__ __forceinline__ void get(float (&fr)[19], float (&fb)[19]) const {
+  __device__ __forceinline__ void get(const LevelIO<S>&, float (&fr)[19], float (&fb)[19]) const {
 #pragma unroll
     for (int l = 0; l < 19; ++l) {
       fr[l] = r[l];
@@ -185,7 +203,7 @@ struct Ahead<S, true> {
 #pragma unroll
     for (int l = 0; l < 19; ++l) {
       q[l] = src.ld(l, n);
-      cvt_raw<S>(q[l], fr[l], fb[l]);
+      src.cvt(l, q[l], fr[l], fb[l]);
     }
     rr = fr[0];
     rb = fb[0];
@@ -197,9 +215,9 @@ struct Ahead<S, true> {
     rr = rr + rho0;
     rb = rb + rho0;
   }
-  __device__ __forceinline__ void get(float (&fr)[19], float (&fb)[19]) const {
+  __device__ __forceinline__ void get(const LevelIO<S>& src, float (&fr)[19], float (&fb)[19]) const {
 #pragma unroll
-    for (int l = 0; l < 19; ++l) cvt_raw<S>(q[l], fr[l], fb[l]);
+    for (int l = 0; l < 19; ++l) src.cvt(l, q[l], fr[l], fb[l]);
   }
 };
 
@@ -304,8 +322,8 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
   const uint32_t pb = (uint32_t)(a.N * (2 * sizeof(S)));   /* bytes of one direction plane */
   const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
   const float rho0 = a.rho0;
-  const LevelIO<S> src(a.src, pb);
-  const LevelIO<S> dst(a.dst, pb);
+  const LevelIO<S> src(a.src, pb, a.rdec, a.renc);
+  const LevelIO<S> dst(a.dst, pb, a.rdec, a.renc);
   const int x0 = itx * TX, y0 = ity * TY, z0 = izc * zc;
   const int z1 = min(z0 + zc, nz);
   const int w = WX ? TX : min(TX, nx - x0);
@@ -388,12 +406,13 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
     }
   };
 
+  bool rest_bad = false;   /* a rest pair beyond its 16-bit range */
   plane(z0 - 1, 0);
   plane(z0, 1);
   int sm = 0, s0 = 1, sp = 2;
   for (int z = z0; z < z1; ++z) {
     float fr[19], fb[19];
-    ahead.get(fr, fb);
+    ahead.get(src, fr, fb);
     const float rR = qR, rB = qB;
     const uint8_t f = qF;
     plane(z + 1, sp);
@@ -419,6 +438,7 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
         }
         collide_node<kShifted<S>>(fr, fb, rR, rB, sphi[s0][ty + 1][c0], px, py, pz, a);
       }
+      if (kShifted<S>) rest_bad |= rest_out_of_range(fr[0], fb[0], a.renc);
       /* push: every node but spheredead sends its 18 moving populations to
          x + e_l and keeps the rest population (flipflop*Pop0_CG, stream*_CG,
          kernels_fluid_CG.CUF:866-1015); non-fluid nodes send their stored
@@ -439,6 +459,7 @@ __device__ __forceinline__ void fused_tile(const FluidArgs& a, int itx, int ity,
     s0 = sp;
     sp = so;
   }
+  if (rest_bad) raise_error(a.status, LBM_ERR_STORE_RANGE, a.step);
 }
 
 template <class S, bool PART, int TX, int TY, bool WX>

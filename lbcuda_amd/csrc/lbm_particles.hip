@@ -83,6 +83,7 @@ struct PartArgs {
   void* pop_src;           /* state buffer of this step          */
   void* pop_dst;           /* next buffer (after collide)        */
   float rho0;
+  float rdec, renc;        /* rest-pair scales (16-bit storage)  */
   float radius;
   int lrotate;
   int wetting;
@@ -205,13 +206,13 @@ __device__ void omega_lab(const double* pos, int p, double o[3]) {
 /* rho_k = f_0 + ... + f_18 (init_rhoR_CG order, :427-436), clamped at
  * MINDENS = 0 (:443-446) */
 template <class S>
-__device__ void rho_sum(const void* pops, size_t N, size_t n, float rho0, float* rr, float* rb) {
+__device__ void rho_sum(const void* pops, size_t N, size_t n, float rho0, float rdec, float* rr, float* rb) {
   float r, b;
-  ld_pair<S>(pops, N, 0, n, rho0, r, b);
+  ld_pair<S>(pops, N, 0, n, rho0, rdec, r, b);
 #pragma unroll
   for (int l = 1; l < 19; ++l) {
     float pr, pb;
-    ld_pair<S>(pops, N, l, n, rho0, pr, pb);
+    ld_pair<S>(pops, N, l, n, rho0, rdec, pr, pb);
     r = r + pr;
     b = b + pb;
   }
@@ -223,7 +224,7 @@ __device__ void rho_sum(const void* pops, size_t N, size_t n, float rho0, float*
 template <class S>
 __device__ __forceinline__ void belt_rho(const PartArgs& a, size_t m, float* rr, float* rb) {
   if (a.rho_from_pops) {
-    rho_sum<S>(a.pop_src, a.N, m, a.rho0, rr, rb);
+    rho_sum<S>(a.pop_src, a.N, m, a.rho0, a.rdec, rr, rb);
   } else {
     *rr = a.rho_r[m];
     *rb = a.rho_b[m];
@@ -290,7 +291,7 @@ __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
        not store them, so they are summed from the previous level */
     float rR, rB;
     if (a.lag_from_pops) {
-      rho_sum<S>(a.pop_dst, a.N, n, a.rho0, &rR, &rB);
+      rho_sum<S>(a.pop_dst, a.N, n, a.rho0, a.rdec, &rR, &rB);
     } else {
       rR = a.rho_r[n];
       rB = a.rho_b[n];
@@ -298,7 +299,7 @@ __device__ __forceinline__ void k_erase_body(const PartArgs& a, int p) {
     const float inv = 1.0f / (rR + rB);
     float fr[19], fb[19];
 #pragma unroll
-    for (int l = 0; l < 19; ++l) ld_pair<S>(a.pop_src, a.N, l, n, a.rho0, fr[l], fb[l]);
+    for (int l = 0; l < 19; ++l) ld_pair<S>(a.pop_src, a.N, l, n, a.rho0, a.rdec, fr[l], fb[l]);
     const float u = inv * (fr[1] - fr[2] + fr[7] - fr[8] - fr[9] + fr[10] + fr[11] - fr[12] - fr[13] + fr[14]) +
                     inv * (fb[1] - fb[2] + fb[7] - fb[8] - fb[9] + fb[10] + fb[11] - fb[12] - fb[13] + fb[14]);
     const float v = inv * (fr[3] - fr[4] + fr[7] - fr[8] + fr[9] - fr[10] + fr[15] - fr[16] - fr[17] + fr[18]) +
@@ -401,7 +402,7 @@ __device__ __forceinline__ void k_fill_apply_body(const PartArgs& a, int p) {
       for (int l = 0; l < 19; ++l) {
         const float uv = (1.0f / CSSQ) * (vx * (float)EX(l) + vy * (float)EY(l) + vz * (float)EZ(l));
         const float brk = 1.0f + uv + 0.5f * (uv * uv) - usq;
-        st_pair<S>(a.pop_src, a.N, l, n, Rs * W(l) * brk, Bs * W(l) * brk, a.rho0);
+        st_pair<S>(a.pop_src, a.N, l, n, Rs * W(l) * brk, Bs * W(l) * brk, a.rho0, a.renc);
       }
     }
     double ft[3] = {(double)(-(Rs + Bs) * vx), (double)(-(Rs + Bs) * vy), (double)(-(Rs + Bs) * vz)};
@@ -566,7 +567,7 @@ __device__ __forceinline__ void shell_ring(const PartArgs& a, const int c[3], ui
   const size_t n = node_at(a, c[0], c[1], c[2], off_c(w, 0), off_c(w, 1), off_c(w, 2));
   const uint8_t f = a.fl_cur[n], fo = a.fl_old[n];
   float rr, rb;
-  rho_sum<S>(a.pop_src, a.N, n, a.rho0, &rr, &rb);   /* issued with the type loads */
+  rho_sum<S>(a.pop_src, a.N, n, a.rho0, a.rdec, &rr, &rb);   /* issued with the type loads */
   if (f != FL_FLUID || fo != FL_FLUID) return;
   a.rho_r[n] = rr;
   a.rho_b[n] = rb;
@@ -644,7 +645,7 @@ __device__ __forceinline__ void shell_ring_box(const PartArgs& a, const int c[3]
   const size_t n = node_at(a, c[0], c[1], c[2], off_c(w, 0), off_c(w, 1), off_c(w, 2));
   const uint8_t f = a.fl_cur[n], fo = a.fl_old[n];
   float rr, rb;
-  rho_sum<S>(a.pop_src, a.N, n, a.rho0, &rr, &rb);   /* issued with the type loads */
+  rho_sum<S>(a.pop_src, a.N, n, a.rho0, a.rdec, &rr, &rb);   /* issued with the type loads */
   const int b = bx.at(off_c(w, 0), off_c(w, 1), off_c(w, 2));
   bx.fl[b] = f;
   if (f != FL_FLUID) return;
@@ -810,7 +811,7 @@ struct LinkData {
       nb[il] = (uint32_t)wrap_node(a, gx + EX(lin), gy + EY(lin), gz + EZ(lin));
       rr[il] = a.rho_r[nb[il]];
       rb[il] = a.rho_b[nb[il]];
-      ld_pair<S>(a.pop_dst, a.N, lout, s, a.rho0, pr[il], pb[il]);
+      ld_pair<S>(a.pop_dst, a.N, lout, s, a.rho0, a.rdec, pr[il], pb[il]);
     }
   }
 };
@@ -945,7 +946,7 @@ __device__ __forceinline__ void k_links_body(const PartArgs& a, int p) {
     for (int il = 1; il <= 9; ++il) {
       if (!LinkData<S>::is_link(m, il)) continue;
       const int lin = LinkData<S>::lo_is_fluid(m, il) ? 2 * il - 1 : 2 * il;
-      st_pair<S>(a.pop_dst, a.N, lin, ld.nb[il], fR[il], fB[il], a.rho0);
+      st_pair<S>(a.pop_dst, a.N, lin, ld.nb[il], fR[il], fB[il], a.rho0, a.renc);
     }
   }
   block_reduce6(own);
@@ -1353,6 +1354,8 @@ PartArgs make_pargs(lbm_ctx* c) {
   a.pop_src = c->pop[c->cur];
   a.pop_dst = c->pop[1 - c->cur];
   a.rho0 = c->storage == LBM_STORE_F32 ? 0.0f : c->cfg.store_rho0;
+  a.rdec = std::ldexp(1.0f, 118 + rest_offset(c->cfg));
+  a.renc = std::ldexp(1.0f, -118 - rest_offset(c->cfg));
   a.radius = c->cfg.radius;
   a.lrotate = c->cfg.lrotate;
   a.wetting = c->cfg.wetting;

@@ -108,9 +108,11 @@ def test_c2_spinodal_128_against_oracle_100_steps(spinodal_128):
               flush=True)
         errs[storage] = (rel, l2)
     # FP32: rounding over 100 steps of a spinodal field (4.0e-5 measured).
-    # FP16: the largest error over 2.1 M nodes is 2.0e-3 (1e-3 holds at 32^3,
-    # test_low_precision_storage_error); the rest population (|f - w rho0| up
-    # to rho0/3) carries most of the storage rounding.  BF16: 1.4e-2.
+    # FP16: L2 1.4e-4; the largest error over 2.1 M nodes is 2.1e-3 (1e-3
+    # holds at 32^3, test_low_precision_storage_error): the spinodal
+    # instability amplifies the storage rounding of the early, mixed field,
+    # and which node carries the largest error changes with the seed
+    # (5.4e-3 for seed 7, L2 unchanged).  BF16: 1.7e-2 max, 1.1e-3 L2.
     assert errs[STORE_F32][0] < 1e-4
     assert errs[STORE_F16][0] < 3e-3 and errs[STORE_F16][1] < 1e-3
     assert errs[STORE_BF16][0] < 2e-2
@@ -145,7 +147,7 @@ def test_c2_spinodal_128_5000_steps(spinodal_128):
     # 16-bit storage: rounding noise makes a colour's density dip below zero
     # where the other phase fills the node, and the MINDENS clamp of the
     # density pass (kernels_fluid_CG.CUF:443-446) turns that noise into a mass
-    # gain: 8.1e-4 (FP16) and 2.1e-2 (BF16) after 5000 steps
+    # gain: 4.9e-4 (FP16) and 5.3e-3 (BF16) after 5000 steps
     assert drift[STORE_F16] < 2e-3 and drift[STORE_BF16] < 5e-2
     # the three runs coarsen alike (the trajectories are chaotic, their
     # statistics are not)

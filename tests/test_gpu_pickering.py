@@ -96,10 +96,10 @@ def test_pickering_256_fp32_and_fp16_against_fp64_oracle():
     # FP32: 5.5e-5 / 2.2e-6 (density), 5.8e-7 (positions), 1.7e-6 (velocities)
     assert res[STORE_F32][0] < 1e-4 and res[STORE_F32][4] < 1e-4 and res[STORE_F32][1] < 1e-5
     assert res[STORE_F32][2] < 1e-5 and res[STORE_F32][3] < 1e-4
-    # FP16: L2 1.9e-4, the same as the spinodal decomposition without spheres
-    # (1.6e-4 at 128^3); the largest node errors (8.8e-3 in the bulk,
+    # FP16: L2 1.7e-4, the same as the spinodal decomposition without spheres
+    # (1.4e-4 at 128^3); the largest node errors (1.0e-2 in the bulk,
     # 1.65e-2 next to spheres) sit where interfaces and moving boundaries
-    # amplify the storage rounding.  Positions 4.9e-4 of a 0.74 displacement,
+    # amplify the storage rounding.  Positions 4.6e-4 of a 0.74 displacement,
     # velocities 2.8e-3 of the largest.
     assert res[STORE_F16][1] < 1e-3 and res[STORE_F16][0] < 1.5e-2 and res[STORE_F16][4] < 3e-2
     assert res[STORE_F16][2] < 2e-3 and res[STORE_F16][3] < 1e-2
