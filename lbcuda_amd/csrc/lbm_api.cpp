@@ -300,7 +300,7 @@ int lbm_destroy(lbm_ctx* c) {
   }
   dfree(c->rho_r); dfree(c->rho_b); dfree(c->phi);
   dfree(c->d_status);
-  dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
+  dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2); dfree(c->d_restamp);
   dfree(c->d_claim); dfree(c->d_tmpl); dfree(c->d_ball); dfree(c->d_ring); dfree(c->d_tlist);
   dfree(c->d_nbr); dfree(c->d_nnbr); dfree(c->d_contrib); dfree(c->d_smask); dfree(c->d_closure);
   for (int e = 0; e < 2 * c->ev_cap; ++e) (void)hipEventDestroy(c->ev_pairs[e]);
@@ -382,6 +382,8 @@ int lbm_set_particles(lbm_ctx* c, int32_t n, const double* pos) {
       dfree(c->d_pos[b]); dfree(c->d_force[b]); dfree(c->d_stamp[b]);
       c->d_pos[b] = nullptr; c->d_force[b] = nullptr; c->d_stamp[b] = nullptr;
     }
+    dfree(c->d_restamp);
+    c->d_restamp = nullptr;
     dfree(c->d_myf); dfree(c->d_myf2); dfree(c->d_myt); dfree(c->d_myt2);
     c->d_myf = c->d_myf2 = c->d_myt = c->d_myt2 = nullptr;
     int rc = 0;
@@ -397,6 +399,7 @@ int lbm_set_particles(lbm_ctx* c, int32_t n, const double* pos) {
       rc |= dalloc(&c->d_force[b], (size_t)12 * n);
       rc |= dalloc(&c->d_stamp[b], (size_t)3 * n);
     }
+    rc |= dalloc(&c->d_restamp, (size_t)n);
     rc |= dalloc(&c->d_myf, (size_t)3 * n);
     rc |= dalloc(&c->d_myf2, (size_t)3 * n);
     rc |= dalloc(&c->d_myt, (size_t)3 * n);

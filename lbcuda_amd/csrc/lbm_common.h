@@ -353,6 +353,7 @@ struct lbm_ctx {
   double* d_myt;
   double* d_myt2;
   int* d_stamp[2];        /* (3, np) centres stamped into flags[b]             */
+  uint8_t* d_restamp;     /* (np) particles the flag pass re-stamps this step  */
   uint32_t* d_key[2];     /* (N) owner key per time level                      */
   int* d_claim;           /* (N) make_fluid writer resolution                  */
   int* d_nbr;             /* (32, np) neighbour lists of the link kernel        */

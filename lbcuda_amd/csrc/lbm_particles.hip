@@ -55,6 +55,7 @@ struct PartArgs {
   const uint8_t* fl_old;
   uint32_t* key_cur;
   int* stamp_cur;          /* (3, np)                            */
+  uint8_t* restamp;        /* (np) re-stamped by this flag pass  */
   int* claim;
   const int8_t* tmpl;      /* (4, ntmpl)                         */
   int ntmpl;
@@ -235,9 +236,48 @@ __device__ __forceinline__ void belt_rho(const PartArgs& a, size_t m, float* rr,
 /* Kernels without per-particle sums spread one particle's offset list over
  * gridDim.y workgroups (blockIdx.y = chunk), so large spheres (4x resolved
  * pair runs) do not serialise on one workgroup. */
+/* template boxes (offsets within rmax per axis) centred at c1 and c2 share a
+ * node only if they meet on every axis (minimum image) */
+__device__ __forceinline__ bool boxes_meet(const PartArgs& a, const int c1[3], const int c2[3]) {
+  const int n[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int d = c1[k] - c2[k];
+    if (2 * d > n[k]) d -= n[k];
+    else if (2 * d < -n[k]) d += n[k];
+    if (d > 2 * a.rmax || d < -2 * a.rmax) return false;
+  }
+  return true;
+}
+
+/* The flag level written this step was stamped two steps ago, at the centres
+ * stamp_cur records.  A particle whose nearest node is the same keeps its
+ * nodes, types and owner keys as they stand, unless a particle that moved
+ * clears some of them (its old template) or claims some (its new one): this
+ * pass clears the old template of each moved particle and marks for k_stamp
+ * and k_resolve the moved particles and those whose template box meets a
+ * moved particle's old or new box.  Any other particle's nodes hold what
+ * re-stamping would write again (the owner is the lowest index either way). */
 __device__ __forceinline__ void k_unstamp_body(const PartArgs& a, int p) {
   const int cx = a.stamp_cur[3 * p], cy = a.stamp_cur[3 * p + 1], cz = a.stamp_cur[3 * p + 2];
-  if (cx == INT32_MIN) return;
+  int c[3];
+  centre(a.pos_cur, p, c);
+  const bool moved_p = cx != c[0] || cy != c[1] || cz != c[2];
+  if (blockIdx.y == 0) {
+    bool hit = false;
+    if (!moved_p) {
+      for (int q = threadIdx.x; q < a.np; q += TPB) {
+        int cq[3];
+        centre(a.pos_cur, q, cq);
+        const int sq[3] = {a.stamp_cur[3 * q], a.stamp_cur[3 * q + 1], a.stamp_cur[3 * q + 2]};
+        if (sq[0] == cq[0] && sq[1] == cq[1] && sq[2] == cq[2]) continue;   /* q kept its node */
+        if (boxes_meet(a, c, cq) || (sq[0] != INT32_MIN && boxes_meet(a, c, sq))) hit = true;
+      }
+    }
+    hit = __syncthreads_or(hit);
+    if (threadIdx.x == 0) a.restamp[p] = (uint8_t)(moved_p || hit);
+  }
+  if (!moved_p || cx == INT32_MIN) return;
   for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
     size_t n = node_at(a, cx, cy, cz, a.tmpl[4 * t], a.tmpl[4 * t + 1], a.tmpl[4 * t + 2]);
     a.fl_cur[n] = FL_FLUID;
@@ -246,6 +286,7 @@ __device__ __forceinline__ void k_unstamp_body(const PartArgs& a, int p) {
 }
 
 __device__ __forceinline__ void k_stamp_body(const PartArgs& a, int p) {
+  if (!a.restamp[p]) return;
   int c[3];
   centre(a.pos_cur, p, c);
   for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
@@ -255,6 +296,7 @@ __device__ __forceinline__ void k_stamp_body(const PartArgs& a, int p) {
 }
 
 __device__ __forceinline__ void k_resolve_body(const PartArgs& a, int p) {
+  if (!a.restamp[p]) return;
   int c[3];
   centre(a.pos_cur, p, c);
   for (int t = blockIdx.y * TPB + threadIdx.x; t < a.ntmpl; t += TPB * gridDim.y) {
@@ -1343,6 +1385,7 @@ PartArgs make_pargs(lbm_ctx* c) {
   a.fl_old = c->flags[1 - c->cur];
   a.key_cur = c->d_key[c->cur];
   a.stamp_cur = c->d_stamp[c->cur];
+  a.restamp = c->d_restamp;
   a.claim = c->d_claim;
   a.tmpl = c->d_tmpl; a.ntmpl = c->ntmpl;
   a.ring = c->d_ring; a.nring = c->nring;
