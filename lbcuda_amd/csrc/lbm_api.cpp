@@ -846,13 +846,23 @@ int lbm_autotune(lbm_ctx* c, int32_t reps, int32_t* tile_out, int32_t* zc_out, d
   double best = 1e30;
   int bt = -1, bz = 0;
   const bool parts = c->np > 0;
-  for (int t = 0; !rc && t < lbm_fused_num_tiles(); ++t) {
+  /* every candidate is timed in two passes over the whole list and keeps
+     its faster pass, so a drift of the clock or memory rate during the sweep
+     does not favour the candidates timed early or late */
+  std::vector<std::pair<int, int>> cand;
+  for (int t = 0; t < lbm_fused_num_tiles(); ++t) {
     int last_zc = -1;
     for (int zc0 : zcs) {
       const int zc = zc0 > c->nz ? c->nz : zc0;
       if (zc == last_zc) continue;
       last_zc = zc;
-      if (!lbm_fused_tile_valid(c, t, zc)) continue;
+      if (lbm_fused_tile_valid(c, t, zc)) cand.push_back({t, zc});
+    }
+  }
+  std::vector<double> tmin(cand.size(), 1e30);
+  for (int pass = 0; !rc && pass < 2; ++pass) {
+    for (size_t k = 0; !rc && k < cand.size(); ++k) {
+      const int t = cand[k].first, zc = cand[k].second;
       rc |= lbm_launch_fused_with(c, t, zc, scratch);   /* warm-up */
       rc |= hip_ok(hipEventRecord(e0, c->stream));
       for (int r = 0; r < reps; ++r) rc |= lbm_launch_fused_with(c, t, zc, scratch);
@@ -860,11 +870,11 @@ int lbm_autotune(lbm_ctx* c, int32_t reps, int32_t* tile_out, int32_t* zc_out, d
       rc |= hip_ok(hipEventSynchronize(e1));
       float ms = 0.0f;
       rc |= hip_ok(hipEventElapsedTime(&ms, e0, e1));
-      if (rc) break;
-      const double per = (double)ms / reps;
-      if (per < best) { best = per; bt = t; bz = zc; }
+      if (!rc) tmin[k] = std::min(tmin[k], (double)ms / reps);
     }
   }
+  for (size_t k = 0; !rc && k < cand.size(); ++k)
+    if (tmin[k] < best) { best = tmin[k]; bt = cand[k].first; bz = cand[k].second; }
   (void)parts;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
